@@ -1,0 +1,137 @@
+"""ctypes binding of the in-tree C ABI (include/bridct.h, include/bridct_gpu.h).
+
+This is how Python code that wants device-resident buffers (benchmarks, batch
+pipelines) reaches the library; array users should prefer the NumPy extension
+``bridct_amd.bridct_numpy``.  Loading fails loudly when the library has not been built:
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libbridct_amd.so')
+
+FORWARD, INVERSE, ROUNDTRIP, FORWARD_THEN_INVERSE = 0, 1, 2, 3
+OK, INVALID_ARGUMENT, WORKSPACE_TOO_SMALL, BAD_ALIGNMENT, OVERLAPPING_BUFFERS, DEVICE_ERROR = (
+    0, -1, -2, -3, -4, -5)
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+# name -> (restype, argtypes); every function declared in include/bridct.h and bridct_gpu.h
+SIGNATURES = {
+    'bridct_supported': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    'bridct_plan_create': (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int]),
+    'bridct_plan_destroy': (None, [ctypes.c_void_p]),
+    'bridct_plan_route': (ctypes.c_char_p, [ctypes.c_void_p]),
+    'bridct_plan_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    'bridct_workspace_floats': (ctypes.c_size_t, [ctypes.c_int]),
+    'bridct_apply': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_gpu_available': (ctypes.c_int, []),
+    'bridct_plan_apply_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_plan_apply_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_plan_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_plan_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
+    'bridct_gpu_malloc': (ctypes.c_void_p, [ctypes.c_size_t]),
+    'bridct_gpu_free': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_gpu_upload': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_gpu_download': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+}
+
+_lib = None
+
+
+def load():
+    """Returns the loaded library with typed entry points (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError('libbridct_amd.so is not built; run __graft_entry__.build() '
+                              'or make -C bridct_amd')
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def ptr(a):
+    """Host pointer of a NumPy array as c_void_p."""
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceBuffer:
+    """Owning device allocation of `nbytes` bytes."""
+
+    def __init__(self, nbytes):
+        self.lib = load()
+        self.nbytes = int(nbytes)
+        self.ptr = self.lib.bridct_gpu_malloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError('bridct_gpu_malloc(%d) failed' % self.nbytes)
+
+    def upload(self, a):
+        assert a.nbytes <= self.nbytes and a.flags.c_contiguous
+        rc = self.lib.bridct_gpu_upload(self.ptr, ptr(a), a.nbytes)
+        if rc:
+            raise RuntimeError('upload failed: %d' % rc)
+
+    def download(self, a):
+        assert a.nbytes <= self.nbytes and a.flags.c_contiguous
+        rc = self.lib.bridct_gpu_download(ptr(a), self.ptr, a.nbytes)
+        if rc:
+            raise RuntimeError('download failed: %d' % rc)
+
+    def free(self):
+        if self.ptr:
+            self.lib.bridct_gpu_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Owning wrapper of a bridct_plan for device-pointer calls."""
+
+    def __init__(self, h, w):
+        self.lib = load()
+        self.h, self.w = h, w
+        self.ptr = self.lib.bridct_plan_create(h, w)
+        if not self.ptr:
+            raise RuntimeError('bridct_plan_create(%d, %d) failed' % (h, w))
+
+    def apply_device(self, mode, d_in, d_out, batch):
+        return self.lib.bridct_plan_apply_device(self.ptr, mode, d_in, d_out, batch)
+
+    def apply_batch(self, mode, x, y):
+        batch = x.size // (self.h * self.w)
+        return self.lib.bridct_plan_apply_batch(self.ptr, mode, ptr(x), ptr(y), batch)
+
+    def apply(self, mode, x, y):
+        return self.lib.bridct_plan_apply(self.ptr, mode, ptr(x), ptr(y))
+
+    def synchronize(self):
+        return self.lib.bridct_plan_synchronize(self.ptr)
+
+    @property
+    def stream(self):
+        return self.lib.bridct_plan_stream(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.lib.bridct_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

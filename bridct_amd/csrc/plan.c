@@ -1,0 +1,324 @@
+/* Host side of the MI355X BRiDCT plan API (C11).
+ *
+ * Mirrors the reference plan lifecycle (src/plan.c:12-163, src/bridct.c:13-75): the same
+ * shape check, route strings, status codes, validation order and "validate before any
+ * write" rule.  The arithmetic is issued to the GPU through the thin C ABI of
+ * bridct_device.h; this file never calls HIP directly.
+ *
+ * A plan owns: one HIP stream, a device block of constant tables (twiddles, tangents and
+ * the per-axis normalization for every axis length 8..1024, evaluated in FP64 and rounded
+ * once to FP32), and device staging buffers for the host-pointer calls.  It has no
+ * process-global mutable state (reference docs/API.md:23).
+ */
+#include "bridct.h"
+#include "bridct_gpu.h"
+
+#include "bridct_device.h"
+#include "generated/sj_factors.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double PI = 3.14159265358979323846;
+
+/* Floats of staging per direction (host-pointer calls are copied in chunks of arrays). */
+#define STAGE_FLOATS ((size_t)1 << 20)
+
+struct bridct_plan {
+    int height, width;
+    size_t elements;      /* h * w */
+    size_t chunk;         /* arrays per staging chunk */
+    void *stream;         /* hipStream_t */
+    float *d_tables;      /* BDX_TABLE_FLOATS */
+    float *d_in, *d_out;  /* chunk * elements each */
+};
+
+/* src/plan.c:33-37 */
+int bridct_supported(int height, int width)
+{
+    return height >= 8 && height <= 1024 && width >= 8 && width <= 1024 &&
+           !(height & (height - 1)) && !(width & (width - 1));
+}
+
+/* Johnson-Frigo scale s_{N,k}: 1 for N <= 4, else s_{N/4, k mod N/4} times the cosine
+ * (k mod N/4 <= N/8) or sine of 2 pi (k mod N/4) / N (src/plan.c:39-47). */
+static double jf_scale(int n, int k)
+{
+    if (n <= 4)
+        return 1.0;
+    k %= n / 4;
+    const double angle = 2.0 * PI * k / n;
+    return jf_scale(n / 4, k) * (k <= n / 8 ? cos(angle) : sin(angle));
+}
+
+static const double *sj_factor(int n)
+{
+    switch (n) {
+    case 8: return bridct_sj_factor_8;
+    case 16: return bridct_sj_factor_16;
+    case 32: return bridct_sj_factor_32;
+    default: return bridct_sj_factor_64;
+    }
+}
+
+/* Per-axis normalization of the GPU core for axis length n, output index k, in FP64:
+ * the orthonormal factor sqrt((k ? 2 : 1) / n) times the core's own output scale.
+ *  - n <= 64: the core is diag(1/d_n) F_n with F_n = C_II / s_{4n,k} (src/plan.c:96),
+ *    so the factor is s_{4n,k} d_n[k] sqrt(...).
+ *  - n >= 128: the core returns C_II[k] / sigma_k with sigma_k = cos(pi k' / (2n)),
+ *    k' = min(k, n - k), halved except at k in {0, n/4, n/2, 3n/4}. */
+static double axis_alpha(int n, int k)
+{
+    const double q = sqrt((k ? 2.0 : 1.0) / n);
+    if (n <= 64)
+        return q * jf_scale(4 * n, k) * sj_factor(n)[k];
+    const int kk = k <= n / 2 ? k : n - k;
+    const int special = k == 0 || k == n / 2 || k == n / 4 || k == 3 * n / 4;
+    return q * cos(PI * kk / (2.0 * n)) * (special ? 1.0 : 0.5);
+}
+
+static void build_tables(float *t)
+{
+    for (int n = 8; n <= 1024; n *= 2) {
+        float *b = t + BDX_TABLE_BASE(n);
+        for (int k = 0; k < n; k++) {
+            const double a = 2.0 * PI * k / n;
+            b[2 * k] = (float)cos(a);
+            b[2 * k + 1] = (float)-sin(a);
+        }
+        for (int k = 0; k < n; k++)
+            b[2 * n + k] = k <= n / 2 ? (float)tan(PI * k / (2.0 * n)) : 0.0f;
+        for (int k = 0; k < n; k++) {
+            const double a = axis_alpha(n, k);
+            b[3 * n + k] = (float)a;
+            b[4 * n + k] = (float)(a * a);
+        }
+    }
+}
+
+void bridct_plan_destroy(bridct_plan *plan)
+{
+    if (!plan)
+        return;
+    if (plan->stream)
+        bdx_stream_sync(plan->stream);
+    if (plan->d_tables)
+        bdx_free(plan->d_tables);
+    if (plan->d_in)
+        bdx_free(plan->d_in);
+    if (plan->d_out)
+        bdx_free(plan->d_out);
+    if (plan->stream)
+        bdx_stream_destroy(plan->stream);
+    free(plan);
+}
+
+bridct_plan *bridct_plan_create(int height, int width)
+{
+    if (!bridct_supported(height, width) || bdx_device_ready() != 0)
+        return NULL;
+    bridct_plan *plan = calloc(1, sizeof(*plan));
+    if (!plan)
+        return NULL;
+    plan->height = height;
+    plan->width = width;
+    plan->elements = (size_t)height * width;
+    plan->chunk = STAGE_FLOATS / plan->elements;
+    if (plan->chunk < 1)
+        plan->chunk = 1;
+    float *host = malloc(BDX_TABLE_FLOATS * sizeof(float));
+    if (!host)
+        goto fail;
+    build_tables(host);
+    const size_t stage = plan->chunk * plan->elements * sizeof(float);
+    if (bdx_stream_create(&plan->stream) ||
+        bdx_malloc((void **)&plan->d_tables, BDX_TABLE_FLOATS * sizeof(float)) ||
+        bdx_malloc((void **)&plan->d_in, stage) || bdx_malloc((void **)&plan->d_out, stage) ||
+        bdx_copy_h2d(plan->d_tables, host, BDX_TABLE_FLOATS * sizeof(float), plan->stream) ||
+        bdx_stream_sync(plan->stream)) {
+        free(host);
+        goto fail;
+    }
+    free(host);
+    return plan;
+
+fail:
+    bridct_plan_destroy(plan);
+    return NULL;
+}
+
+/* src/plan.c:117-124 without the x86-only AVX2 special case: the categories are
+ * historical shape classes (docs/API.md:23). */
+const char *bridct_plan_route(const bridct_plan *plan)
+{
+    if (!plan)
+        return NULL;
+    return plan->height == plan->width && plan->height <= 256 ? "paper8-square" : "blocked-extension";
+}
+
+static int distance_below(const void *a, const void *b, size_t bytes)
+{
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return (x <= y ? y - x : x - y) < bytes;
+}
+
+static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+{
+    if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
+        return bdx_transform(d_in, d_out, plan->height, plan->width, count, BDX_FORWARD,
+                             plan->d_tables, plan->stream) ||
+               bdx_transform(d_out, d_out, plan->height, plan->width, count, BDX_INVERSE,
+                             plan->d_tables, plan->stream);
+    return bdx_transform(d_in, d_out, plan->height, plan->width, count, mode, plan->d_tables,
+                         plan->stream);
+}
+
+static int run_host(bridct_plan *plan, int mode, const float *input, float *output, size_t batch)
+{
+    for (size_t done = 0; done < batch;) {
+        size_t count = batch - done;
+        if (count > plan->chunk)
+            count = plan->chunk;
+        const size_t floats = count * plan->elements;
+        if (bdx_copy_h2d(plan->d_in, input + done * plan->elements, floats * sizeof(float),
+                         plan->stream) ||
+            transform(plan, mode, plan->d_in, plan->d_out, count) ||
+            bdx_copy_d2h(output + done * plan->elements, plan->d_out, floats * sizeof(float),
+                         plan->stream) ||
+            bdx_stream_sync(plan->stream))
+            return BRIDCT_DEVICE_ERROR;
+        done += count;
+    }
+    return BRIDCT_OK;
+}
+
+/* src/plan.c:126-135: argument checks, then the overlap rule (distance below h*w*4). */
+int bridct_plan_apply(bridct_plan *plan, enum bridct_mode mode, const float *input, float *output)
+{
+    if (!plan || !input || !output || (unsigned)mode > BRIDCT_ROUNDTRIP)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (distance_below(input, output, plan->elements * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    return run_host(plan, (int)mode, input, output, 1);
+}
+
+int bridct_plan_apply_batch(bridct_plan *plan, int mode, const float *input, float *output,
+                            size_t batch)
+{
+    if (!plan || !input || !output || (unsigned)mode > BRIDCT_GPU_FORWARD_THEN_INVERSE || batch < 1 ||
+        batch > SIZE_MAX / sizeof(float) / plan->elements)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (distance_below(input, output, batch * plan->elements * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    return run_host(plan, mode, input, output, batch);
+}
+
+int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode, const float *d_input,
+                             float *d_output, size_t batch)
+{
+    if (!plan || !d_input || !d_output || (unsigned)mode > BRIDCT_ROUNDTRIP ||
+        batch > SIZE_MAX / sizeof(float) / plan->elements)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (((uintptr_t)d_input | (uintptr_t)d_output) % 16)
+        return BRIDCT_BAD_ALIGNMENT;
+    if (d_input != d_output &&
+        distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    if (bdx_transform(d_input, d_output, plan->height, plan->width, batch, (int)mode,
+                      plan->d_tables, plan->stream))
+        return BRIDCT_DEVICE_ERROR;
+    return BRIDCT_OK;
+}
+
+int bridct_plan_synchronize(bridct_plan *plan)
+{
+    if (!plan)
+        return BRIDCT_INVALID_ARGUMENT;
+    return bdx_stream_sync(plan->stream) ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
+}
+
+void *bridct_plan_stream(const bridct_plan *plan)
+{
+    return plan ? plan->stream : NULL;
+}
+
+/* src/bridct.c:13-21: the caller-workspace size of the legacy square API, kept for
+ * callers that size buffers with it. */
+size_t bridct_workspace_floats(int n)
+{
+    if (n != 8 && n != 16 && n != 32 && n != 64 && n != 128 && n != 256)
+        return 0;
+    const size_t e = (size_t)n * n;
+    return n <= 16 ? 2 * e : 8 * e + 2 * 1100 * 4 + 4 * 4 * 256 + 4 * (size_t)n * (4 * n + 4);
+}
+
+static int overlaps(const void *a, size_t bytes_a, const void *b, size_t bytes_b)
+{
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return x <= y ? y - x < bytes_a : x - y < bytes_b;
+}
+
+/* src/bridct.c:30-45: legacy square transform with caller workspace.  The checks and
+ * their precedence are the reference's; the workspace itself is not needed on the GPU. */
+int bridct_apply(int n, enum bridct_mode mode, const float *input, float *output,
+                 float *workspace, size_t workspace_count)
+{
+    const size_t needed = bridct_workspace_floats(n);
+    if (!needed || !input || !output || !workspace || (unsigned)mode > BRIDCT_ROUNDTRIP)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (workspace_count < needed)
+        return BRIDCT_WORKSPACE_TOO_SMALL;
+    if ((uintptr_t)workspace % 16)
+        return BRIDCT_BAD_ALIGNMENT;
+    const size_t bytes = (size_t)n * n * sizeof(float);
+    if (overlaps(input, bytes, output, bytes) ||
+        overlaps(input, bytes, workspace, needed * sizeof(float)) ||
+        overlaps(output, bytes, workspace, needed * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    bridct_plan *plan = bridct_plan_create(n, n);
+    if (!plan)
+        return BRIDCT_DEVICE_ERROR;
+    const int status = run_host(plan, (int)mode, input, output, 1);
+    bridct_plan_destroy(plan);
+    return status;
+}
+
+int bridct_gpu_available(void)
+{
+    return bdx_device_ready() == 0;
+}
+
+void *bridct_gpu_malloc(size_t bytes)
+{
+    void *p = NULL;
+    if (bdx_device_ready() != 0 || bdx_malloc(&p, bytes ? bytes : 1))
+        return NULL;
+    return p;
+}
+
+int bridct_gpu_free(void *d_ptr)
+{
+    if (!d_ptr)
+        return BRIDCT_OK;
+    return bdx_free(d_ptr) ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
+}
+
+int bridct_gpu_upload(void *d_dst, const void *src, size_t bytes)
+{
+    if (!d_dst || !src)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (bdx_copy_h2d(d_dst, src, bytes, NULL) || bdx_stream_sync(NULL))
+        return BRIDCT_DEVICE_ERROR;
+    return BRIDCT_OK;
+}
+
+int bridct_gpu_download(void *dst, const void *d_src, size_t bytes)
+{
+    if (!dst || !d_src)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (bdx_copy_d2h(dst, d_src, bytes, NULL) || bdx_stream_sync(NULL))
+        return BRIDCT_DEVICE_ERROR;
+    return BRIDCT_OK;
+}

@@ -1,0 +1,302 @@
+/* bridct_numpy: CPython/NumPy interface to the MI355X BRiDCT plan library.
+ *
+ * Same module, type and contract as the reference extension (python/bridct_numpy.c):
+ *   Plan(height, width); forward / inverse / roundtrip(input, *, out=None); close();
+ *   getters shape, route, closed; module constant core_version.
+ * Inputs are exact float32 ndarrays of shape (h, w) or (B, h, w), native byte order,
+ * C-contiguous and aligned; `out` must also be writable, of the same ndim and size, and
+ * must not overlap the input.  TypeError for type/keyword problems, ValueError for
+ * shape/layout/overlap, RuntimeError for closed/busy/re-initialized plans and failed
+ * calls, MemoryError for allocation.  roundtrip materializes the normalized FP32
+ * spectrum (python/bridct_numpy.c:114-133), so it equals inverse(forward(x)) bitwise.
+ * The GIL is released for inputs of 16384 elements or more, and a per-plan busy flag
+ * rejects concurrent use (python/bridct_numpy.c:142-147, 191-198).
+ *
+ * The whole batch goes to the GPU in one call (bridct_plan_apply_batch), not one call
+ * per image as in the reference loop (python/bridct_numpy.c:114-133).
+ */
+#define PY_SSIZE_T_CLEAN
+#define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
+#include <Python.h>
+#include <numpy/arrayobject.h>
+
+#include "bridct.h"
+#include "bridct_gpu.h"
+
+#include <stdint.h>
+
+#ifdef Py_GIL_DISABLED
+#error "bridct_numpy needs a CPython build with the GIL."
+#endif
+
+typedef struct {
+    PyObject_HEAD
+    bridct_plan *plan;
+    size_t elements;
+    int height, width, initialized, busy;
+} Plan;
+
+static int Plan_init(Plan *self, PyObject *args, PyObject *kwargs)
+{
+    static char *keywords[] = {"height", "width", NULL};
+    int height, width;
+    if (self->initialized) {
+        PyErr_SetString(PyExc_RuntimeError, "A Plan cannot be reinitialized.");
+        return -1;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This Plan is already initializing or executing.");
+        return -1;
+    }
+    self->busy = 1;
+    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "ii:Plan", keywords, &height, &width))
+        goto fail;
+    if (!bridct_supported(height, width)) {
+        PyErr_SetString(PyExc_ValueError, "Both dimensions must be powers of two from 8 through 1024.");
+        goto fail;
+    }
+    if (!bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT GPU plan cannot be created.");
+        goto fail;
+    }
+    bridct_plan *plan = bridct_plan_create(height, width);
+    if (!plan) {
+        PyErr_NoMemory();
+        goto fail;
+    }
+    self->plan = plan;
+    self->elements = (size_t)height * (size_t)width;
+    self->height = height;
+    self->width = width;
+    self->initialized = 1;
+    self->busy = 0;
+    return 0;
+fail:
+    self->busy = 0;
+    return -1;
+}
+
+static void Plan_dealloc(Plan *self)
+{
+    bridct_plan_destroy(self->plan);
+    Py_TYPE(self)->tp_free((PyObject *)self);
+}
+
+static PyObject *Plan_close(Plan *self, PyObject *unused)
+{
+    (void)unused;
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError,
+                        "This Plan is already initializing or executing, possibly in another thread.");
+        return NULL;
+    }
+    bridct_plan_destroy(self->plan);
+    self->plan = NULL;
+    Py_RETURN_NONE;
+}
+
+static PyArrayObject *validated(PyObject *object, const Plan *self, int writable)
+{
+    if (!PyArray_CheckExact(object)) {
+        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+        return NULL;
+    }
+    PyArrayObject *a = (PyArrayObject *)object;
+    if (PyArray_TYPE(a) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(a)) {
+        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32.");
+        return NULL;
+    }
+    const int nd = PyArray_NDIM(a);
+    if ((nd != 2 && nd != 3) || PyArray_DIM(a, nd - 2) != self->height ||
+        PyArray_DIM(a, nd - 1) != self->width || (nd == 3 && PyArray_DIM(a, 0) < 1)) {
+        PyErr_SetString(PyExc_ValueError, "Expected (height, width) or (batch, height, width), with batch >= 1.");
+        return NULL;
+    }
+    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
+        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, float-aligned storage.");
+        return NULL;
+    }
+    if (writable && !PyArray_ISWRITEABLE(a)) {
+        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
+        return NULL;
+    }
+    return a;
+}
+
+static PyObject *execute(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
+                         int mode)
+{
+    if (!self->plan) {
+        PyErr_SetString(PyExc_RuntimeError, "This Plan is closed or uninitialized.");
+        return NULL;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This Plan is executing in another thread; use one Plan per thread.");
+        return NULL;
+    }
+    /* Array allocation can run Python code; the flag covers setup as well as compute. */
+    self->busy = 1;
+    PyArrayObject *out = NULL;
+    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
+    if (nargs != 1 || nkw > 1) {
+        PyErr_SetString(PyExc_TypeError, "Use method(input, *, out=None).");
+        goto fail;
+    }
+    PyObject *out_obj = Py_None;
+    if (nkw) {
+        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
+            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
+            goto fail;
+        }
+        out_obj = args[1];
+    }
+    PyArrayObject *in = validated(args[0], self, 0);
+    if (!in)
+        goto fail;
+    if (out_obj == Py_None) {
+        out = (PyArrayObject *)PyArray_SimpleNew(PyArray_NDIM(in), PyArray_DIMS(in), NPY_FLOAT32);
+        if (!out)
+            goto fail;
+    } else {
+        PyArrayObject *given = validated(out_obj, self, 1);
+        if (!given)
+            goto fail;
+        if (PyArray_NDIM(in) != PyArray_NDIM(given) || PyArray_SIZE(in) != PyArray_SIZE(given)) {
+            PyErr_SetString(PyExc_ValueError, "Input and output shapes must match exactly.");
+            goto fail;
+        }
+        const uintptr_t a = (uintptr_t)PyArray_DATA(in), b = (uintptr_t)PyArray_DATA(given);
+        if ((a <= b ? b - a : a - b) < (size_t)PyArray_NBYTES(in)) {
+            PyErr_SetString(PyExc_ValueError, "Input and output must not overlap.");
+            goto fail;
+        }
+        out = given;
+        Py_INCREF(out);
+    }
+    const float *x = PyArray_DATA(in);
+    float *y = PyArray_DATA(out);
+    const size_t batch = PyArray_NDIM(in) == 3 ? (size_t)PyArray_DIM(in, 0) : 1;
+    int status;
+    if (PyArray_SIZE(in) >= 16384) {
+        Py_BEGIN_ALLOW_THREADS
+        status = bridct_plan_apply_batch(self->plan, mode, x, y, batch);
+        Py_END_ALLOW_THREADS
+    } else {
+        status = bridct_plan_apply_batch(self->plan, mode, x, y, batch);
+    }
+    if (status) {
+        PyErr_Format(PyExc_RuntimeError, "BRiDCT failed with status %d.", status);
+        goto fail;
+    }
+    self->busy = 0;
+    return (PyObject *)out;
+fail:
+    self->busy = 0;
+    Py_XDECREF(out);
+    return NULL;
+}
+
+static PyObject *Plan_forward(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return execute(self, args, nargs, kw, BRIDCT_FORWARD);
+}
+
+static PyObject *Plan_inverse(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return execute(self, args, nargs, kw, BRIDCT_INVERSE);
+}
+
+static PyObject *Plan_roundtrip(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return execute(self, args, nargs, kw, BRIDCT_GPU_FORWARD_THEN_INVERSE);
+}
+
+static PyObject *Plan_shape(Plan *self, void *unused)
+{
+    (void)unused;
+    return Py_BuildValue("(ii)", self->height, self->width);
+}
+
+static PyObject *Plan_route(Plan *self, void *unused)
+{
+    (void)unused;
+    if (!self->plan)
+        Py_RETURN_NONE;
+    return PyUnicode_FromString(bridct_plan_route(self->plan));
+}
+
+static PyObject *Plan_closed(Plan *self, void *unused)
+{
+    (void)unused;
+    return PyBool_FromLong(self->plan == NULL);
+}
+
+#define AS_METHOD(fn) (PyCFunction)(void (*)(void))(fn)
+static PyMethodDef Plan_methods[] = {
+    {"forward", AS_METHOD(Plan_forward), METH_FASTCALL | METH_KEYWORDS,
+     "forward(input, *, out=None): orthonormal 2D DCT-II over the last two axes, on the GPU."},
+    {"inverse", AS_METHOD(Plan_inverse), METH_FASTCALL | METH_KEYWORDS,
+     "inverse(input, *, out=None): orthonormal 2D DCT-III over the last two axes, on the GPU."},
+    {"roundtrip", AS_METHOD(Plan_roundtrip), METH_FASTCALL | METH_KEYWORDS,
+     "roundtrip(input, *, out=None): forward then inverse with a materialized spectrum."},
+    {"close", AS_METHOD(Plan_close), METH_NOARGS, "Release the GPU plan and its buffers."},
+    {NULL, NULL, 0, NULL}};
+
+static PyGetSetDef Plan_getset[] = {
+    {"shape", (getter)Plan_shape, NULL, "Planned (height, width).", NULL},
+    {"route", (getter)Plan_route, NULL, "Route category, or None when closed.", NULL},
+    {"closed", (getter)Plan_closed, NULL, "Whether the plan has been closed.", NULL},
+    {NULL, NULL, NULL, NULL, NULL}};
+
+static PyTypeObject PlanType = {
+    PyVarObject_HEAD_INIT(NULL, 0)
+    .tp_name = "bridct_numpy.Plan",
+    .tp_basicsize = sizeof(Plan),
+    .tp_dealloc = (destructor)Plan_dealloc,
+    .tp_flags = Py_TPFLAGS_DEFAULT,
+    .tp_doc = "Reusable orthonormal float32 2D DCT plan on the GPU; one plan per concurrent worker.",
+    .tp_methods = Plan_methods,
+    .tp_getset = Plan_getset,
+    .tp_init = (initproc)Plan_init,
+    .tp_new = PyType_GenericNew};
+
+static int module_exec(PyObject *m)
+{
+    if (PyInterpreterState_Get() != PyInterpreterState_Main()) {
+        PyErr_SetString(PyExc_ImportError, "bridct_numpy supports the main interpreter only.");
+        return -1;
+    }
+    if (_import_array() < 0)
+        return -1;
+    if (PyType_Ready(&PlanType) < 0)
+        return -1;
+    Py_INCREF(&PlanType);
+    if (PyModule_AddObject(m, "Plan", (PyObject *)&PlanType) < 0) {
+        Py_DECREF(&PlanType);
+        return -1;
+    }
+    if (PyModule_AddStringConstant(m, "core_version", BRIDCT_VERSION) < 0)
+        return -1;
+    return 0;
+}
+
+#if defined(__GNUC__) && !defined(__clang__)
+#pragma GCC diagnostic push
+#pragma GCC diagnostic ignored "-Wpedantic"
+#endif
+static PyModuleDef_Slot module_slots[] = {{Py_mod_exec, (void *)module_exec}, {0, NULL}};
+#if defined(__GNUC__) && !defined(__clang__)
+#pragma GCC diagnostic pop
+#endif
+
+static PyModuleDef module_def = {
+    PyModuleDef_HEAD_INIT,
+    .m_name = "bridct_numpy",
+    .m_doc = "NumPy interface to the MI355X BRiDCT 2D DCT plan library.",
+    .m_size = 0,
+    .m_slots = module_slots};
+
+PyMODINIT_FUNC PyInit_bridct_numpy(void)
+{
+    return PyModuleDef_Init(&module_def);
+}

@@ -1,0 +1,55 @@
+/* GPU extensions of the BRiDCT plan API (MI355X).  No counterpart in the reference
+ * interface; they expose what a device-resident caller needs: device pointers, batches,
+ * streams, and a few device-memory helpers so that C and Python callers need no other
+ * GPU runtime binding.
+ */
+#ifndef BRIDCT_GPU_H
+#define BRIDCT_GPU_H
+
+#include "bridct.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status of a failed GPU runtime call (allocation, copy, launch). */
+#define BRIDCT_DEVICE_ERROR (-5)
+
+/* 1 when a gfx950 device is present and usable, else 0. */
+BRIDCT_API int bridct_gpu_available(void);
+
+/* Applies the plan to `batch` contiguous h x w arrays in device memory on the plan's
+ * stream, asynchronously.  d_input and d_output must be 16-byte aligned and either the
+ * same pointer (in place) or disjoint.  Statuses as bridct_plan_apply, plus
+ * BRIDCT_BAD_ALIGNMENT and BRIDCT_DEVICE_ERROR. */
+BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode,
+                                        const float *d_input, float *d_output, size_t batch);
+
+/* Mode value accepted by bridct_plan_apply_batch only: FORWARD then INVERSE with the
+ * normalized spectrum materialized in FP32 between them (the NumPy Plan.roundtrip
+ * semantics of the reference, python/bridct_numpy.c:114-133).  Bitwise equal to the two
+ * separate calls. */
+#define BRIDCT_GPU_FORWARD_THEN_INVERSE 3
+
+/* Host-pointer batch: `batch` contiguous arrays, copied through the plan's device
+ * buffers in chunks.  Same validation as bridct_plan_apply over the whole batch;
+ * mode may also be BRIDCT_GPU_FORWARD_THEN_INVERSE. */
+BRIDCT_API int bridct_plan_apply_batch(bridct_plan *plan, int mode, const float *input,
+                                       float *output, size_t batch);
+
+/* Waits for all work issued on the plan's stream. */
+BRIDCT_API int bridct_plan_synchronize(bridct_plan *plan);
+
+/* The plan's stream (a hipStream_t), for callers that order their own work after it. */
+BRIDCT_API void *bridct_plan_stream(const bridct_plan *plan);
+
+/* Device memory helpers (synchronous). */
+BRIDCT_API void *bridct_gpu_malloc(size_t bytes);
+BRIDCT_API int bridct_gpu_free(void *d_ptr);
+BRIDCT_API int bridct_gpu_upload(void *d_dst, const void *src, size_t bytes);
+BRIDCT_API int bridct_gpu_download(void *dst, const void *d_src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+#!/usr/bin/env python3
+"""Writes the committed golden fixtures of the 2D DCT plan path from the reference tree.
+
+Run in the build container (the reference tree is not present where GPU tests run):
+    python tests/golden/make_golden.py [--reference /root/reference]
+
+It reads only data and literal tables from the reference, and runs only the reference
+library compiled from its own sources by oracle/Makefile (oracle/_ref/libbridct.so):
+
+* axis_scale.json: the per-axis orthonormal scales scale8..scale1024 printed in the
+  reference's generated tables (src/generated/rect_generated.inc:115, 350, 834, 1695,
+  2878, 4629, 6929, 9858).  They equal s_{4n,k} sqrt((k ? 2 : 1)/n) and pin the oracle's
+  sj_scale / axis_scale.
+* corpus_small.npz: the 8x8 and 16x16 core cases of the reference corpus
+  (dataset/inputs.npz, sha256-checked against dataset/manifest.json) with the reference
+  CPU outputs for FORWARD, INVERSE and ROUNDTRIP.
+* corpus_summary.json: for all 474 corpus inputs and 3 modes, the reference CPU output's
+  input-relative Frobenius error against the FP64 Q-matrix closed form
+  (dataset/evaluate.py:17-21), and the reference's published maximum over core cases
+  (dataset/DATASET_CARD.md:34) for comparison.
+"""
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, 'tests'))
+
+
+def parse_scales(ref):
+    text = open(os.path.join(ref, 'src/generated/rect_generated.inc')).read()
+    out = {}
+    for m in re.finditer(r'static const float scale(\d+)\[\]=\{([^}]*)\}', text):
+        n = int(m.group(1))
+        vals = [float(v.rstrip('f')) for v in m.group(2).split(',')]
+        assert len(vals) == n
+        out[str(n)] = vals
+    assert sorted(map(int, out)) == [8, 16, 32, 64, 128, 256, 512, 1024]
+    return out
+
+
+def error_norm(actual, expected, input_norm):
+    """dataset/evaluate.py:17-21 restated."""
+    if not np.isfinite(actual).all():
+        return None
+    error = float(np.linalg.norm(actual.astype(np.float64) - expected))
+    return error / input_norm if input_norm else (0.0 if error == 0.0 else None)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--reference', default='/root/reference')
+    args = ap.parse_args()
+    import support as S
+    if not S.ref_available():
+        raise SystemExit('build oracle/_ref first: make -C oracle')
+
+    scales = parse_scales(args.reference)
+    with open(os.path.join(HERE, 'axis_scale.json'), 'w') as f:
+        json.dump({'source': 'src/generated/rect_generated.inc scale{n}[]', 'scale': scales}, f)
+
+    ds = os.path.join(args.reference, 'dataset')
+    manifest = json.load(open(os.path.join(ds, 'manifest.json')))
+    archive = os.path.join(ds, 'inputs.npz')
+    assert hashlib.sha256(open(archive, 'rb').read()).hexdigest() == manifest['archive_sha256']
+    rows = []
+    small = {}
+    with np.load(archive, allow_pickle=False) as data:
+        for case in manifest['cases']:
+            x = data[case['id']]
+            assert hashlib.sha256(x.tobytes(order='C')).hexdigest() == case['sha256']
+            n = case['N']
+            q = S.qmatrix(n)
+            xd = x.astype(np.float64)
+            norm = float(np.linalg.norm(xd))
+            outs = []
+            for mode, exact in enumerate((q @ xd @ q.T, q.T @ xd @ q, xd)):
+                with np.errstate(all='ignore'):
+                    y = S.ref_apply(x, mode)
+                    e = error_norm(y, exact, norm)
+                outs.append(y)
+                rows.append(dict(id=case['id'], n=n, domain=case['domain'], mode=mode, error=e))
+            if n <= 16 and case['domain'] == 'core':
+                small[case['id']] = (x, outs)
+    core = [r['error'] for r in rows if r['domain'] == 'core' and r['error'] is not None]
+    summary = dict(
+        cases=len(manifest['cases']), directions=len(rows),
+        core_directions=sum(r['domain'] == 'core' for r in rows),
+        core_max_error=max(core),
+        published_core_max_error=2.250789485770909e-7,
+        boundary_fail=sum(1 for r in rows if r['domain'] != 'core' and
+                          (r['error'] is None or r['error'] > 2e-5)),
+        published_boundary_fail=90,
+        rows=rows)
+    with open(os.path.join(HERE, 'corpus_summary.json'), 'w') as f:
+        json.dump(summary, f, indent=0)
+    ids = sorted(small)
+    np.savez_compressed(
+        os.path.join(HERE, 'corpus_small.npz'),
+        ids=np.array(ids),
+        x=np.stack([small[i][0] for i in ids if small[i][0].shape[0] == 8]),
+        x16=np.stack([small[i][0] for i in ids if small[i][0].shape[0] == 16]),
+        **{'y%d_8' % m: np.stack([small[i][1][m] for i in ids if small[i][0].shape[0] == 8])
+           for m in range(3)},
+        **{'y%d_16' % m: np.stack([small[i][1][m] for i in ids if small[i][0].shape[0] == 16])
+           for m in range(3)})
+    print('core max error %.6e (published %.6e), boundary failures %d (published 90)'
+          % (summary['core_max_error'], 2.250789485770909e-7, summary['boundary_fail']))
+
+
+if __name__ == '__main__':
+    main()

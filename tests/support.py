@@ -1,0 +1,144 @@
+"""Shared test helpers: the oracle (FP64 closed form), the reference CPU library built from
+source under oracle/_ref (when present), error metrics and the reference's input families.
+
+Only tests (and the smoke check / bench CPU-baseline leg) load anything under oracle/.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, 'oracle', 'liboracle.so')
+REF_LIB = os.path.join(ROOT, 'oracle', '_ref', 'libbridct.so')
+LENGTHS = (8, 16, 32, 64, 128, 256, 512, 1024)
+SHAPES = [(h, w) for h in LENGTHS for w in LENGTHS]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = ctypes.CDLL(ORACLE_LIB)
+        lib.oracle_supported.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.oracle_sj_scale.restype = ctypes.c_double
+        lib.oracle_sj_scale.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.oracle_axis_scale.restype = ctypes.c_double
+        lib.oracle_axis_scale.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.oracle_norm_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.oracle_qmatrix.argtypes = [ctypes.c_int, ctypes.c_void_p]
+        lib.oracle_core.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.oracle_dct2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        _oracle = lib
+    return _oracle
+
+
+def qmatrix(n):
+    q = np.empty((n, n), np.float64)
+    assert oracle().oracle_qmatrix(n, q.ctypes.data) == 0
+    return q
+
+
+_qcache = {}
+
+
+def oracle_2d(x, mode):
+    """FP64 orthonormal 2D transform of x (..., h, w) through the oracle's Q matrices:
+    mode 0: Q_h X Q_w^T, 1: Q_h^T X Q_w, 2: X (reference docs/API.md:9-12)."""
+    x = np.asarray(x, np.float64)
+    h, w = x.shape[-2:]
+    if mode == 2:
+        return x.copy()
+    for n in (h, w):
+        if n not in _qcache:
+            _qcache[n] = qmatrix(n)
+    qh, qw = _qcache[h], _qcache[w]
+    if mode == 0:
+        return qh @ x @ qw.T
+    return qh.T @ x @ qw
+
+
+def oracle_2d_direct(x, mode):
+    """The oracle's own C loop (oracle_dct2d), for one h x w array."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    h, w = x.shape
+    assert oracle().oracle_dct2d(h, w, mode, x.ctypes.data, y.ctypes.data) == 0
+    return y
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """Reference BRiDCT CPU library built from the reference sources (oracle/Makefile)."""
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_LIB)
+        lib.bridct_plan_create.restype = ctypes.c_void_p
+        lib.bridct_plan_create.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.bridct_plan_destroy.argtypes = [ctypes.c_void_p]
+        lib.bridct_plan_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.bridct_plan_route.restype = ctypes.c_char_p
+        lib.bridct_plan_route.argtypes = [ctypes.c_void_p]
+        _ref = lib
+    return _ref
+
+
+def ref_apply(x, mode):
+    """Reference CPU output for (..., h, w) float32 input (mode 3: forward then inverse)."""
+    x = np.ascontiguousarray(x, np.float32)
+    h, w = x.shape[-2:]
+    lib = ref()
+    plan = lib.bridct_plan_create(h, w)
+    assert plan
+    flat = x.reshape(-1, h, w)
+    out = np.empty_like(flat)
+    spec = np.empty((h, w), np.float32)
+    try:
+        for i in range(flat.shape[0]):
+            if mode == 3:
+                assert lib.bridct_plan_apply(plan, 0, flat[i].ctypes.data, spec.ctypes.data) == 0
+                assert lib.bridct_plan_apply(plan, 1, spec.ctypes.data, out[i].ctypes.data) == 0
+            else:
+                assert lib.bridct_plan_apply(plan, mode, flat[i].ctypes.data, out[i].ctypes.data) == 0
+    finally:
+        lib.bridct_plan_destroy(plan)
+    return out.reshape(x.shape)
+
+
+def max_rel(y, ref):
+    """max|y - ref| / max|ref| (infinity-norm relative error)."""
+    d = np.abs(np.asarray(y, np.float64) - ref).max()
+    m = np.abs(ref).max()
+    return float(d / m) if m else float(d)
+
+
+def frob_input_rel(y, ref, x):
+    """||y - ref||_F / ||x||_F (reference dataset/evaluate.py:17-21)."""
+    err = float(np.linalg.norm(np.asarray(y, np.float64) - ref))
+    n = float(np.linalg.norm(np.asarray(x, np.float64)))
+    return err / n if n else (0.0 if err == 0.0 else float('inf'))
+
+
+def rel_l2(y, ref):
+    d = float(np.linalg.norm(np.asarray(y, np.float64) - np.asarray(ref, np.float64)))
+    n = float(np.linalg.norm(np.asarray(ref, np.float64)))
+    return d / n if n else d
+
+
+def families(h, w, rng):
+    """The reference's shape-sweep families (tests/verify_portability.py:64-73)."""
+    noise = rng.normal(size=(h, w)).astype(np.float32)
+    alt = np.where(np.indices((h, w)).sum(axis=0) % 2, 1., -1.)
+    impulse = np.zeros((h, w), np.float32)
+    impulse[h // 3, w // 2] = 1
+    return [('zero', np.zeros_like(impulse)), ('impulse', impulse), ('random', noise),
+            ('neighbors', (1 + alt * 2 ** -23).astype(np.float32)),
+            ('tiny', (noise * 2. ** -70).astype(np.float32)),
+            ('large', (noise * 2. ** 50).astype(np.float32))]
