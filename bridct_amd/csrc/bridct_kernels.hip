@@ -621,13 +621,11 @@ int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, i
         per = 64;
     const size_t rblocks = (nrows + per - 1) / per;
     const size_t rlds = (size_t)per * (w + 1) * 4;
-    int cw = kLdsFloats / h;
-    while ((size_t)h * (cw + 1) > (size_t)kLdsFloats)
+    int cw = 64;   // power of two, so strips tile w exactly and stay 16-byte aligned
+    while (cw > 8 && (size_t)h * (cw + 1) > (size_t)kLdsFloats)
         cw /= 2;
     if (cw > w)
         cw = w;
-    if (cw > 64)
-        cw = 64;
     const size_t cblocks = batch * (size_t)(w / cw);
     const size_t clds = (size_t)h * (cw + 1) * 4;
     const int rop = mode == BDX_FORWARD ? FA : mode == BDX_INVERSE ? IA : FA2;
