@@ -1,0 +1,42 @@
+// Launchers of the per-shape kernels (host side of the HIP translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace bdx {
+
+// Line operations: FA forward * alpha, FA2 forward * alpha^2, IA alpha * inverse, I0 inverse.
+enum Op { FA = 1, FA2 = 2, IA = 3, I0 = 4 };
+
+constexpr int kThreads = 256;
+// Floats of LDS for one workgroup of the whole-array kernel (144 KiB of the 160 KiB).
+constexpr int kSmallLds = 36864;
+// Floats of LDS per workgroup of the row / column pass kernels (two workgroups per CU).
+constexpr int kPassLds = 16384;
+
+// Whole-array ("small") route: h * (w + 1) <= kSmallLds.
+constexpr bool small_route(int h, int w) { return (long)h * (w + 1) <= kSmallLds; }
+
+// Rows per block of the row pass and columns per strip of the column pass.
+constexpr int rows_per_block(int w) { return kPassLds / w; }
+constexpr int cols_per_block(int h) { return kPassLds / h > 256 ? 256 : kPassLds / h; }
+
+// mode: 0 forward, 1 inverse, 2 fused round trip.  Return 0 or a negative value.
+int launch_small_h8(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h16(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h32(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h64(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h128(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h256(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h512(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h1024(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+
+// Row pass over nrows rows of length w: op 1 = forward*alpha, 2 = forward*alpha^2,
+// 3 = alpha*inverse, 4 = inverse.
+int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s);
+// Column pass over `batch` h x w arrays, in place allowed: mode 0 forward*alpha,
+// 1 alpha*inverse, 2 forward*alpha^2 then inverse.
+int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
+                hipStream_t s);
+
+}  // namespace bdx

@@ -1,0 +1,116 @@
+// Thin C ABI of the HIP side (declared in bridct_device.h): device checks, memory, streams
+// and the route choice of one batched transform.
+//
+// Route per shape (the GPU replacement of rect_policy.inc / the square-variant table of
+// src/bridct.c:47-74):
+//   * h * (w + 1) floats fit 144 KiB of LDS: one whole-array kernel (bdx_small.hip);
+//   * otherwise: row pass (input -> output) then column pass in place on the output
+//     (bdx_passes.hip); the fused round trip adds a third, inverse row pass.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "bdx_kernels.h"
+#include "bridct_device.h"
+
+namespace {
+
+int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
+                   hipStream_t s)
+{
+    using namespace bdx;
+    switch (h) {
+    case 8: return launch_small_h8(w, in, out, batch, mode, tab, s);
+    case 16: return launch_small_h16(w, in, out, batch, mode, tab, s);
+    case 32: return launch_small_h32(w, in, out, batch, mode, tab, s);
+    case 64: return launch_small_h64(w, in, out, batch, mode, tab, s);
+    case 128: return launch_small_h128(w, in, out, batch, mode, tab, s);
+    case 256: return launch_small_h256(w, in, out, batch, mode, tab, s);
+    case 512: return launch_small_h512(w, in, out, batch, mode, tab, s);
+    case 1024: return launch_small_h1024(w, in, out, batch, mode, tab, s);
+    default: return -1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bdx_device_ready(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return -1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return -1;
+    return strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 0 : -2;
+}
+
+int bdx_malloc(void **ptr, size_t bytes)
+{
+    return hipMalloc(ptr, bytes) == hipSuccess ? 0 : -1;
+}
+
+int bdx_free(void *ptr)
+{
+    return hipFree(ptr) == hipSuccess ? 0 : -1;
+}
+
+int bdx_stream_create(void **stream)
+{
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        return -1;
+    *stream = (void *)s;
+    return 0;
+}
+
+int bdx_stream_destroy(void *stream)
+{
+    return hipStreamDestroy((hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_stream_sync(void *stream)
+{
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_copy_h2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_copy_d2h(void *dst, const void *src, size_t bytes, void *stream)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
+                  const float *d_tables, void *stream)
+{
+    if (batch == 0)
+        return 0;
+    if (mode < BDX_FORWARD || mode > BDX_ROUNDTRIP)
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (bdx::small_route(h, w))
+        return small_dispatch(h, w, d_in, d_out, batch, mode, d_tables, s);
+    const size_t nrows = batch * (size_t)h;
+    const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;
+    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s) ||
+        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s))
+        return -1;
+    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s))
+        return -1;
+    return 0;
+}
+
+}  // extern "C"
