@@ -51,7 +51,7 @@ BDX_SJ(32)
 BDX_SJ(64)
 #undef BDX_SJ
 
-template <int n> constexpr int team_size() { return n <= 64 ? 1 : n >= 1024 ? 16 : n >= 256 ? 8 : 4; }
+template <int n> constexpr int team_size() { return BDX_TEAM(n); }
 
 // Lanes of one team live in one wave; LDS traffic between them needs ordering only
 // within the wave.

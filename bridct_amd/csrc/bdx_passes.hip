@@ -24,16 +24,20 @@ template <int W, int OP>
 __global__ __launch_bounds__(kThreads) void k_rows_t(const float *in, float *out,
                                                       const float *__restrict__ tab)
 {
-    constexpr int T = team_size<W>(), R = kThreads / T;
-    __shared__ float xl[R * W];
+    using R = RowTab<W>;
+    constexpr int T = team_size<W>(), TEAMS = kThreads / T;
+    __shared__ float lt[R::SIZE];
+    __shared__ float xl[TEAMS * R::XCH];
+    load_row_tables<W, OP, kThreads>(lt, tab);
+    __syncthreads();
     const int team = threadIdx.x / T, t = threadIdx.x % T;
-    const size_t row = (size_t)blockIdx.x * R + team;
+    const size_t row = (size_t)blockIdx.x * TEAMS + team;
     const float *src = in + row * W;
     float *dst = out + row * W;
     if constexpr (OP == FA || OP == FA2)
-        rteam_fwd<W, OP>(src, dst, t, xl + team * W, tab);
+        rteam_fwd<W, OP>(src, dst, t, xl + team * R::XCH, lt);
     else
-        rteam_inv<W, OP>(src, dst, t, xl + team * W, tab);
+        rteam_inv<W, OP>(src, dst, t, xl + team * R::XCH, lt);
 }
 
 template <int OP>
@@ -60,11 +64,12 @@ __global__ __launch_bounds__(64 * team_size<H>()) void k_cols_w(const float *in,
     const int strips = w >> 6;
     const size_t a = blockIdx.x / strips;
     const int c0 = (int)(blockIdx.x - a * strips) * 64;
-    const size_t off = a * (size_t)H * w + c0 + lane;
+    // uniform row base (scalar registers) plus the lane's column offset
+    const size_t off = a * (size_t)H * w + c0;
     const float *src = in + off;
     float *dst = out + off;
-    const auto ld = [&](int i) { return src[(size_t)i * w]; };
-    const auto st = [&](int i, float v) { dst[(size_t)i * w] = v; };
+    const auto ld = [&](int i) { return (src + (size_t)i * w)[lane]; };
+    const auto st = [&](int i, float v) { (dst + (size_t)i * w)[lane] = v; };
     float o[Q][T][4];
     if constexpr (MODE == 1) {
         const float *al = alpha_of<H>(tab, 1);

@@ -13,9 +13,12 @@ extern "C" {
 #endif
 
 /* Number of floats of the constant-table block a plan uploads (all axis lengths 8..1024). */
-#define BDX_TABLE_FLOATS (5 * (2048 - 8))
-/* Offset of the block for axis length n: omega (2n), tan (n), alpha (n), alpha^2 (n). */
-#define BDX_TABLE_BASE(n) (5 * ((n) - 8))
+#define BDX_TABLE_FLOATS (6 * (2048 - 8))
+/* Offset of the block for axis length n: omega (2n), tan (n), alpha (n), alpha^2 (n),
+ * team twiddles (n: complex exp(-2 pi i t k1 / (n/2)) at 2 (k1 T + t), T = team size). */
+#define BDX_TABLE_BASE(n) (6 * ((n) - 8))
+/* Team size of the multi-lane 1D transform of length n (1 below 128). */
+#define BDX_TEAM(n) ((n) <= 64 ? 1 : (n) >= 1024 ? 16 : (n) >= 256 ? 8 : 4)
 
 /* Transform modes seen by the kernels (same values as enum bridct_mode). */
 #define BDX_FORWARD 0

@@ -81,6 +81,7 @@ static double axis_alpha(int n, int k)
 
 static void build_tables(float *t)
 {
+    memset(t, 0, BDX_TABLE_FLOATS * sizeof(float));
     for (int n = 8; n <= 1024; n *= 2) {
         float *b = t + BDX_TABLE_BASE(n);
         for (int k = 0; k < n; k++) {
@@ -95,6 +96,14 @@ static void build_tables(float *t)
             b[3 * n + k] = (float)a;
             b[4 * n + k] = (float)(a * a);
         }
+        /* team twiddles exp(-2 pi i t k1 / N), N = n/2, stored at 2 (k1 T + t) */
+        const int T = BDX_TEAM(n), N = n / 2, N1 = N / T;
+        for (int k1 = 0; k1 < N1 && T > 1; k1++)
+            for (int t = 0; t < T; t++) {
+                const double a = 2.0 * PI * (double)((t * k1) % N) / N;
+                b[5 * n + 2 * (k1 * T + t)] = (float)cos(a);
+                b[5 * n + 2 * (k1 * T + t) + 1] = (float)-sin(a);
+            }
     }
 }
 
