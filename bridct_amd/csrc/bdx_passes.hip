@@ -84,20 +84,14 @@ __global__ __launch_bounds__(64 * team_size<H>()) void k_cols_w(const float *in,
                 }
         wteam_inv<H>(t, o, st, xl, tab);
     } else {
-        wteam_fwd<H>(t, ld, xl, tab, o);
         const float *al = alpha_of<H>(tab, MODE == 0 ? 1 : 2);
-#pragma unroll
-        for (int r = 0; r < Q; r++)
-#pragma unroll
-            for (int k2 = 0; k2 < T; k2++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const int idx = wteam_index<H, T>(t + T * r, k2, c);
-                    if constexpr (MODE == 0)
-                        st(idx, o[r][k2][c] * al[idx]);
-                    else
-                        o[r][k2][c] *= al[idx];
-                }
+        wteam_fwd<H>(t, ld, xl, tab, [&](int r, int k2, int c, float v) {
+            const int idx = wteam_index<H, T>(t + T * r, k2, c);
+            if constexpr (MODE == 0)
+                st(idx, v * al[idx]);
+            else
+                o[r][k2][c] = v * al[idx];
+        });
         if constexpr (MODE == 2)
             wteam_inv<H>(t, o, st, xl, tab);
     }

@@ -7,11 +7,13 @@
 // float4s of the mirrored lane T - 1 - t; one DPP mirror (row_mirror for T = 16,
 // row_half_mirror for 8, quad_perm [3,2,1,0] for 4) per value brings them over, with no
 // LDS traffic.  The exchange between the two FFT phases goes through a per-team LDS
-// region of N + T floats, real parts then imaginary parts, with the column index XORed
-// by the row index so that both the writes and the reads of a wave hit distinct banks.
-// Per-lane constants (team twiddles, post-process twiddles and tangents, normalization)
-// come from a per-block LDS copy of the table block (RowTab).  Outputs go straight to
-// global memory.  Algorithm and index maps as in bdx_lines.h and bdx_post.h.
+// region, real parts then imaginary parts, with the column index XORed by the row index
+// so that both the writes and the reads of a wave hit distinct banks.  The forward
+// outputs are scattered over the row, so they are collected in the same region and
+// written back as whole float4 lines (scattered 64-byte stores write partial cache lines
+// and cost extra HBM write traffic).  Per-lane constants (team twiddles, post-process
+// twiddles and tangents, normalization) come from a per-block LDS copy of the table
+// block (RowTab).  Algorithm and index maps as in bdx_lines.h and bdx_post.h.
 #pragma once
 #include "bdx_lines.h"
 #include "bdx_post.h"
@@ -32,7 +34,9 @@ template <int n> struct RowTab {
     static constexpr int TN = 2 * n;            // tan(pi k / (2n)), k <= N
     static constexpr int AL = 2 * n + N + 4;    // normalization (alpha or alpha^2)
     static constexpr int SIZE = AL + n;
-    static constexpr int XCH = N + BDX_TEAM(n); // exchange floats per team (pad spreads teams over banks)
+    // per-team LDS: the FFT exchange (N floats) and then the output row (n floats), padded
+    // so that the teams of one wave start on different banks
+    static constexpr int XCH = n + BDX_TEAM(n);
 };
 
 template <int n, int OP, int THREADS>
@@ -162,10 +166,17 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
                 float y = o[w];
                 if constexpr (OP == FA || OP == FA2)
                     y *= lt[R::AL + idx];
-                dst[idx] = y;
+                xl[idx] = y;
             }
         }
     }
+    team_sync();
+#pragma unroll
+    for (int c = 0; c < n / (4 * T); c++) {
+        const int e = 4 * t + 4 * T * c;
+        *reinterpret_cast<float4 *>(dst + e) = make_float4(xl[e], xl[e + 1], xl[e + 2], xl[e + 3]);
+    }
+    team_sync();
 }
 
 // Inverse: inputs at the lane-owned output indices of the forward, outputs as float4

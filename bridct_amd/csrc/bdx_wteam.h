@@ -33,11 +33,10 @@ __device__ __forceinline__ int wteam_index(int g, int k2, int which)
     return which == 0 ? k : which == 1 ? n - k : which == 2 ? N - k : N + k;
 }
 
-// Forward core.  ld(i) returns element i of the lane's line.  On return o[r][k2][w] holds
-// the unscaled output of index wteam_index(t + T r, k2, w).
-template <int n, typename Ld>
-__device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *__restrict__ tab,
-                                          float (&o)[n / 2 / team_size<n>() / (2 * team_size<n>())][team_size<n>()][4])
+// Forward core.  ld(i) returns element i of the lane's line; emit(r, k2, w, v) receives
+// the unscaled output of index wteam_index(t + T r, k2, w) as soon as it is computed.
+template <int n, typename Ld, typename Emit>
+__device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *__restrict__ tab, Emit emit)
 {
     constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
     constexpr int ROUNDS = wteam_rounds<n>(), SL = 64 / ROUNDS;
@@ -97,26 +96,34 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
         if (g == 0) {
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
+                float o[4];
                 if (k2 == T / 2) {
                     post_fwd_special(dr[r][0][0], di[r][0][0], dr[r][0][T / 2], di[r][0][T / 2],
-                                     tn[n / 4], o[r][k2]);
-                    continue;
+                                     tn[n / 4], o);
+                } else {
+                    const bool lo = k2 < T / 2;
+                    const int j = k2 - T / 2;
+                    const int k = lo ? N1 / 2 + N1 * k2 : N1 * j;
+                    const float ar = lo ? dr[r][1][k2] : dr[r][0][j];
+                    const float ai = lo ? di[r][1][k2] : di[r][0][j];
+                    const float br = lo ? dr[r][1][T - 1 - k2] : dr[r][0][T - j];
+                    const float bi = lo ? di[r][1][T - 1 - k2] : di[r][0][T - j];
+                    post_fwd(ar, ai, br, bi, om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o);
                 }
-                const bool lo = k2 < T / 2;
-                const int j = k2 - T / 2;
-                const int k = lo ? N1 / 2 + N1 * k2 : N1 * j;
-                const float ar = lo ? dr[r][1][k2] : dr[r][0][j];
-                const float ai = lo ? di[r][1][k2] : di[r][0][j];
-                const float br = lo ? dr[r][1][T - 1 - k2] : dr[r][0][T - j];
-                const float bi = lo ? di[r][1][T - 1 - k2] : di[r][0][T - j];
-                post_fwd(ar, ai, br, bi, om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o[r][k2]);
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    emit(r, k2, w, o[w]);
             }
         } else {
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 const int k = g + N1 * k2;
+                float o[4];
                 post_fwd(dr[r][0][k2], di[r][0][k2], dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2],
-                         om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o[r][k2]);
+                         om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o);
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    emit(r, k2, w, o[w]);
             }
         }
     }
