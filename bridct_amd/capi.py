@@ -34,6 +34,18 @@ SIGNATURES = {
                                                ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_plan_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
+    'bridct_batch_create': (ctypes.c_void_p, [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
+    'bridct_batch_destroy': (None, [ctypes.c_void_p]),
+    'bridct_batch_elements': (ctypes.c_size_t, [ctypes.c_void_p]),
+    'bridct_batch_offset': (ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_batch_items': (ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_int]),
+    'bridct_batch_apply_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
+    'bridct_batch_graph_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
+    'bridct_batch_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_batch_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    'bridct_batch_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_gpu_malloc': (ctypes.c_void_p, [ctypes.c_size_t]),
     'bridct_gpu_free': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_gpu_upload': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
@@ -128,6 +140,58 @@ class Plan:
     def close(self):
         if self.ptr:
             self.lib.bridct_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Owning wrapper of a bridct_batch: arrays of shapes `shapes` [(h, w), ...] stored back
+    to back in one buffer."""
+
+    def __init__(self, shapes):
+        import numpy as np
+        self.lib = load()
+        self.shapes = [(int(h), int(w)) for h, w in shapes]
+        hs = np.ascontiguousarray([h for h, _ in self.shapes], dtype=np.int32)
+        ws = np.ascontiguousarray([w for _, w in self.shapes], dtype=np.int32)
+        self.ptr = self.lib.bridct_batch_create(len(self.shapes), ptr(hs), ptr(ws))
+        if not self.ptr:
+            raise RuntimeError('bridct_batch_create failed')
+        self.elements = self.lib.bridct_batch_elements(self.ptr)
+        self.offsets = [self.lib.bridct_batch_offset(self.ptr, i) for i in range(len(self.shapes))]
+
+    def items(self, mode):
+        return self.lib.bridct_batch_items(self.ptr, mode)
+
+    def apply_device(self, mode, d_in, d_out):
+        return self.lib.bridct_batch_apply_device(self.ptr, mode, d_in, d_out)
+
+    def graph_apply(self, mode, d_in, d_out):
+        return self.lib.bridct_batch_graph_apply(self.ptr, mode, d_in, d_out)
+
+    def apply(self, mode, x, y):
+        assert x.size == self.elements and y.size == self.elements
+        return self.lib.bridct_batch_apply(self.ptr, mode, ptr(x), ptr(y))
+
+    def synchronize(self):
+        return self.lib.bridct_batch_synchronize(self.ptr)
+
+    @property
+    def stream(self):
+        return self.lib.bridct_batch_stream(self.ptr)
+
+    def split(self, flat):
+        """Views of the arrays of a flat batch buffer."""
+        return [flat[o:o + h * w].reshape(h, w) for o, (h, w) in zip(self.offsets, self.shapes)]
+
+    def close(self):
+        if self.ptr:
+            self.lib.bridct_batch_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

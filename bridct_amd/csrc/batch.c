@@ -1,0 +1,424 @@
+/* Ragged batches: arrays of mixed supported shapes transformed by one persistent launch
+ * (C11 host side of bdx_batch.hip).
+ *
+ * The reference applies one plan per array shape (src/plan.c:126-163; callers loop over
+ * arrays, e.g. python/bridct_numpy.c:96-112).  A batch object takes the whole list of
+ * shapes once, lays the arrays out back to back, and turns every array into work items
+ * of the persistent kernel:
+ *   - arrays with both axes at most 128: SMALL items of up to BDX_BATCH_PER(h, w) whole
+ *     arrays of one run (a run = consecutive arrays of one shape);
+ *   - larger arrays: ROW items (bands of rows, input -> output), COL items (strips of
+ *     columns, in place, each waiting for all ROW items of its array) and, for the fused
+ *     round trip, ROW2 items (inverse rows, waiting for all COL items).
+ * Queue order: two-pass arrays by decreasing size; the COL items of an array are queued
+ * `lag` items after its last ROW item (and ROW2 after COL likewise), so that the column
+ * strips of one array run while the rows of later arrays are in flight and waits are
+ * short; SMALL items go between the last ROW items and the final COL/ROW2 items.  Every
+ * item only waits for items earlier in the queue, which were taken by workgroups that are
+ * running, so the schedule cannot deadlock.
+ *
+ * The object owns its stream, device tables, item lists and counters, and one captured
+ * graph (hipGraph) for replays of the same (mode, input, output); it has no global state.
+ */
+#include "bridct_gpu.h"
+
+#include "bridct_device.h"
+#include "host_internal.h"
+
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Dependency waits give up (and flag an error) after this many sleeps of the waiting
+ * workgroup, about a second: a legitimate wait is one array's row pass. */
+#define SPIN_LIMIT (1u << 21)
+
+typedef struct {
+    bdx_item *items;
+    unsigned n, cap;
+} item_list;
+
+struct bridct_batch {
+    size_t count, elements;
+    size_t *offsets;         /* element offset of each array */
+    unsigned nruns, ntwo;    /* runs, two-pass arrays */
+    unsigned blocks;
+    void *stream;
+    float *d_tables;
+    bdx_run *d_runs;
+    bdx_item *d_items[2];    /* [0] forward / inverse order, [1] fused round trip */
+    unsigned nitems[2];
+    unsigned *d_ctr;
+    unsigned nctr;
+    float *d_in, *d_out;     /* staging for host-pointer calls, allocated on first use */
+    void *graph;             /* captured exec and the call it replays */
+    int graph_mode;
+    const float *graph_in;
+    float *graph_out;
+    /* arguments of the call being captured */
+    int cap_mode;
+    const float *cap_in;
+    float *cap_out;
+};
+
+static int push(item_list *l, unsigned run, unsigned idx, unsigned unit, unsigned kind, unsigned ctr)
+{
+    if (l->n == l->cap) {
+        const unsigned cap = l->cap ? 2 * l->cap : 1024;
+        bdx_item *p = realloc(l->items, (size_t)cap * sizeof(*p));
+        if (!p)
+            return -1;
+        l->items = p;
+        l->cap = cap;
+    }
+    bdx_item *it = &l->items[l->n++];
+    it->run = run;
+    it->idx = idx;
+    it->unit = unit;
+    it->kind_ctr = kind << 28 | ctr;
+    return 0;
+}
+
+typedef struct {
+    unsigned run, idx;
+    int h, w;
+} two_pass;
+
+static int by_size_desc(const void *a, const void *b)
+{
+    const two_pass *x = a, *y = b;
+    const long sx = (long)x->h * x->w, sy = (long)y->h * y->w;
+    if (sx != sy)
+        return sx < sy ? 1 : -1;
+    /* stable: keep batch order among arrays of one size */
+    if (x->run != y->run)
+        return x->run < y->run ? -1 : 1;
+    return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+
+static int push_units(item_list *l, const two_pass *a, unsigned c, unsigned kind)
+{
+    const unsigned units = kind == BDX_ITEM_COL ? (unsigned)BDX_CEIL(a->w, BDX_COL_UNIT(a->h))
+                                                : (unsigned)BDX_CEIL(a->h, BDX_ROW_UNIT(a->w));
+    for (unsigned u = 0; u < units; u++)
+        if (push(l, a->run, a->idx, u, kind, c))
+            return -1;
+    return 0;
+}
+
+/* Builds the queue of one mode family (fused = round trip with ROW2 items). */
+static int build_items(item_list *l, const bdx_run *runs, unsigned nruns, const two_pass *tp,
+                       unsigned ntwo, int fused, unsigned lag)
+{
+    /* end position (queue index after the last item) of each array's previous stage */
+    unsigned *row_end = malloc((ntwo + 1) * sizeof(unsigned));
+    unsigned *col_end = malloc((ntwo + 1) * sizeof(unsigned));
+    if (!row_end || !col_end) {
+        free(row_end);
+        free(col_end);
+        return -1;
+    }
+    unsigned next_col = 0, next_row2 = 0;
+    int rc = 0;
+    for (unsigned k = 0; k < ntwo && !rc; k++) {
+        while (!rc && next_col < k && l->n >= row_end[next_col] + lag) {
+            rc = push_units(l, &tp[next_col], next_col, BDX_ITEM_COL);
+            col_end[next_col] = l->n;
+            next_col++;
+        }
+        while (!rc && fused && next_row2 < next_col && l->n >= col_end[next_row2] + lag) {
+            rc = push_units(l, &tp[next_row2], next_row2, BDX_ITEM_ROW2);
+            next_row2++;
+        }
+        if (!rc)
+            rc = push_units(l, &tp[k], k, BDX_ITEM_ROW);
+        row_end[k] = l->n;
+    }
+    /* whole arrays: independent fillers before the last dependent items */
+    for (unsigned r = 0; r < nruns && !rc; r++) {
+        const int h = (int)runs[r].h, w = (int)runs[r].w;
+        if (!BDX_BATCH_SMALL(h, w))
+            continue;
+        const unsigned per = (unsigned)BDX_BATCH_PER(h, w), n = runs[r].count;
+        const unsigned parts = (n + per - 1) / per;
+        for (unsigned p = 0, first = 0; p < parts && !rc; p++) {
+            const unsigned take = (n - first) / (parts - p) + ((n - first) % (parts - p) != 0);
+            rc = push(l, r, first, take, BDX_ITEM_SMALL, 0);
+            first += take;
+        }
+    }
+    while (!rc && next_col < ntwo) {
+        rc = push_units(l, &tp[next_col], next_col, BDX_ITEM_COL);
+        col_end[next_col] = l->n;
+        next_col++;
+    }
+    while (!rc && fused && next_row2 < ntwo) {
+        rc = push_units(l, &tp[next_row2], next_row2, BDX_ITEM_ROW2);
+        next_row2++;
+    }
+    free(row_end);
+    free(col_end);
+    return rc;
+}
+
+void bridct_batch_destroy(bridct_batch *b)
+{
+    if (!b)
+        return;
+    if (b->stream)
+        bdx_stream_sync(b->stream);
+    if (b->graph)
+        bdx_graph_destroy(b->graph);
+    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
+        if (bufs[i])
+            bdx_free(bufs[i]);
+    if (b->stream)
+        bdx_stream_destroy(b->stream);
+    free(b->offsets);
+    free(b);
+}
+
+static int upload(void **dst, const void *src, size_t bytes, void *stream)
+{
+    if (bdx_malloc(dst, bytes ? bytes : 16))
+        return -1;
+    return bytes ? bdx_copy_h2d(*dst, src, bytes, stream) : 0;
+}
+
+void bdx_schedule_free(bdx_schedule *s)
+{
+    if (!s)
+        return;
+    free(s->runs);
+    free(s->items[0]);
+    free(s->items[1]);
+    free(s->offsets);
+    memset(s, 0, sizeof(*s));
+}
+
+int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s)
+{
+    memset(s, 0, sizeof(*s));
+    if (count == 0 || count > UINT32_MAX / 2 || !heights || !widths)
+        return -1;
+    for (size_t i = 0; i < count; i++)
+        if (!bridct_supported(heights[i], widths[i]))
+            return -1;
+    two_pass *tp = malloc(count * sizeof(two_pass));
+    item_list lists[2] = {{0}, {0}};
+    s->offsets = malloc(count * sizeof(size_t));
+    s->runs = malloc(count * sizeof(bdx_run));
+    if (!tp || !s->offsets || !s->runs) {
+        free(tp);
+        bdx_schedule_free(s);
+        return -1;
+    }
+    /* runs of consecutive equal shapes; two-pass arrays */
+    size_t off = 0;
+    for (size_t i = 0; i < count; i++) {
+        const int h = heights[i], w = widths[i];
+        s->offsets[i] = off;
+        if (s->nruns && (int)s->runs[s->nruns - 1].h == h && (int)s->runs[s->nruns - 1].w == w) {
+            s->runs[s->nruns - 1].count++;
+        } else {
+            bdx_run *r = &s->runs[s->nruns++];
+            r->h = (unsigned)h;
+            r->w = (unsigned)w;
+            r->count = 1;
+            r->pad = 0;
+            r->off = off;
+        }
+        if (!BDX_BATCH_SMALL(h, w)) {
+            two_pass *a = &tp[s->ntwo++];
+            a->run = s->nruns - 1;
+            a->idx = s->runs[s->nruns - 1].count - 1;
+            a->h = h;
+            a->w = w;
+        }
+        off += (size_t)h * w;
+    }
+    s->count = count;
+    s->elements = off;
+    qsort(tp, s->ntwo, sizeof(two_pass), by_size_desc);
+    const int rc = build_items(&lists[0], s->runs, s->nruns, tp, s->ntwo, 0, lag) ||
+                   build_items(&lists[1], s->runs, s->nruns, tp, s->ntwo, 1, lag);
+    free(tp);
+    s->items[0] = lists[0].items;
+    s->items[1] = lists[1].items;
+    s->nitems[0] = lists[0].n;
+    s->nitems[1] = lists[1].n;
+    s->nctr = BDX_CTR_BASE + 2 * s->ntwo;
+    if (rc) {
+        bdx_schedule_free(s);
+        return -1;
+    }
+    return 0;
+}
+
+bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *widths)
+{
+    if (bdx_device_ready() != 0)
+        return NULL;
+    const int cus = bdx_device_cus();
+    if (cus <= 0)
+        return NULL;
+    bridct_batch *b = calloc(1, sizeof(*b));
+    if (!b)
+        return NULL;
+    /* two 256-lane workgroups per CU (LDS and registers) */
+    b->blocks = 2u * (unsigned)cus;
+    bdx_schedule sc;
+    if (bdx_batch_schedule(count, heights, widths, b->blocks / 2, &sc)) {
+        free(b);
+        return NULL;
+    }
+    b->count = sc.count;
+    b->elements = sc.elements;
+    b->offsets = sc.offsets;
+    sc.offsets = NULL;
+    b->nruns = sc.nruns;
+    b->ntwo = sc.ntwo;
+    b->nitems[0] = sc.nitems[0];
+    b->nitems[1] = sc.nitems[1];
+    b->nctr = sc.nctr;
+    float *host_tables = malloc(BDX_TABLE_FLOATS * sizeof(float));
+    if (!host_tables)
+        goto fail;
+    bridct_build_tables(host_tables);
+    if (bdx_stream_create(&b->stream) ||
+        upload((void **)&b->d_tables, host_tables, BDX_TABLE_FLOATS * sizeof(float), b->stream) ||
+        upload((void **)&b->d_runs, sc.runs, sc.nruns * sizeof(bdx_run), b->stream) ||
+        upload((void **)&b->d_items[0], sc.items[0], sc.nitems[0] * sizeof(bdx_item), b->stream) ||
+        upload((void **)&b->d_items[1], sc.items[1], sc.nitems[1] * sizeof(bdx_item), b->stream) ||
+        bdx_malloc((void **)&b->d_ctr, b->nctr * sizeof(unsigned)) || bdx_stream_sync(b->stream))
+        goto fail;
+    free(host_tables);
+    bdx_schedule_free(&sc);
+    return b;
+
+fail:
+    free(host_tables);
+    bdx_schedule_free(&sc);
+    bridct_batch_destroy(b);
+    return NULL;
+}
+
+size_t bridct_batch_elements(const bridct_batch *b)
+{
+    return b ? b->elements : 0;
+}
+
+size_t bridct_batch_offset(const bridct_batch *b, size_t index)
+{
+    return b && index < b->count ? b->offsets[index] : 0;
+}
+
+size_t bridct_batch_items(const bridct_batch *b, int mode)
+{
+    if (!b || mode < 0 || mode > BRIDCT_GPU_FORWARD_THEN_INVERSE)
+        return 0;
+    return mode == BRIDCT_ROUNDTRIP ? b->nitems[1] : mode == BRIDCT_GPU_FORWARD_THEN_INVERSE ? 2 * b->nitems[0] : b->nitems[0];
+}
+
+static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
+{
+    const int fused = mode == BDX_ROUNDTRIP;
+    return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_ctr, b->nctr,
+                            mode, b->d_tables, b->blocks, SPIN_LIMIT, b->stream);
+}
+
+static int issue(bridct_batch *b, int mode, const float *d_in, float *d_out)
+{
+    if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
+        return launch(b, BDX_FORWARD, d_in, d_out) || launch(b, BDX_INVERSE, d_out, d_out);
+    return launch(b, mode, d_in, d_out);
+}
+
+static int check_device_args(const bridct_batch *b, int mode, const float *d_in, const float *d_out)
+{
+    if (!b || !d_in || !d_out || mode < 0 || mode > BRIDCT_GPU_FORWARD_THEN_INVERSE)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (((uintptr_t)d_in | (uintptr_t)d_out) % 16)
+        return BRIDCT_BAD_ALIGNMENT;
+    const uintptr_t x = (uintptr_t)d_in, y = (uintptr_t)d_out;
+    if (d_in != d_out && (x <= y ? y - x : x - y) < b->elements * sizeof(float))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    return BRIDCT_OK;
+}
+
+int bridct_batch_apply_device(bridct_batch *b, int mode, const float *d_input, float *d_output)
+{
+    const int st = check_device_args(b, mode, d_input, d_output);
+    if (st)
+        return st;
+    return issue(b, mode, d_input, d_output) ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
+}
+
+static int capture_fn(void *arg)
+{
+    bridct_batch *b = arg;
+    return issue(b, b->cap_mode, b->cap_in, b->cap_out);
+}
+
+int bridct_batch_graph_apply(bridct_batch *b, int mode, const float *d_input, float *d_output)
+{
+    const int st = check_device_args(b, mode, d_input, d_output);
+    if (st)
+        return st;
+    if (!b->graph || b->graph_mode != mode || b->graph_in != d_input || b->graph_out != d_output) {
+        if (b->graph) {
+            if (bdx_stream_sync(b->stream))
+                return BRIDCT_DEVICE_ERROR;
+            bdx_graph_destroy(b->graph);
+            b->graph = NULL;
+        }
+        b->cap_mode = mode;
+        b->cap_in = d_input;
+        b->cap_out = d_output;
+        if (bdx_graph_capture(b->stream, capture_fn, b, &b->graph)) {
+            b->graph = NULL;
+            return BRIDCT_DEVICE_ERROR;
+        }
+        b->graph_mode = mode;
+        b->graph_in = d_input;
+        b->graph_out = d_output;
+    }
+    return bdx_graph_launch(b->graph, b->stream) ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
+}
+
+int bridct_batch_synchronize(bridct_batch *b)
+{
+    if (!b)
+        return BRIDCT_INVALID_ARGUMENT;
+    unsigned err = 0;
+    if (bdx_stream_sync(b->stream) ||
+        bdx_copy_d2h(&err, b->d_ctr + BDX_CTR_ERROR, sizeof(err), b->stream) || bdx_stream_sync(b->stream))
+        return BRIDCT_DEVICE_ERROR;
+    return err ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
+}
+
+int bridct_batch_apply(bridct_batch *b, int mode, const float *input, float *output)
+{
+    if (!b || !input || !output || mode < 0 || mode > BRIDCT_GPU_FORWARD_THEN_INVERSE)
+        return BRIDCT_INVALID_ARGUMENT;
+    const size_t bytes = b->elements * sizeof(float);
+    const uintptr_t x = (uintptr_t)input, y = (uintptr_t)output;
+    if ((x <= y ? y - x : x - y) < bytes)
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    if (!b->d_in && (bdx_malloc((void **)&b->d_in, bytes) || bdx_malloc((void **)&b->d_out, bytes))) {
+        if (b->d_in)
+            bdx_free(b->d_in);
+        b->d_in = NULL;
+        return BRIDCT_DEVICE_ERROR;
+    }
+    if (bdx_copy_h2d(b->d_in, input, bytes, b->stream) || issue(b, mode, b->d_in, b->d_out) ||
+        bdx_copy_d2h(output, b->d_out, bytes, b->stream))
+        return BRIDCT_DEVICE_ERROR;
+    return bridct_batch_synchronize(b);
+}
+
+void *bridct_batch_stream(const bridct_batch *b)
+{
+    return b ? b->stream : NULL;
+}

@@ -1,0 +1,214 @@
+// Persistent batch kernel: ragged batches of arrays of mixed supported shapes in one
+// launch.  Compiled once per transform mode BDX_MODE (0 forward, 1 inverse, 2 fused
+// round trip).
+//
+// BDX_BATCH_THREADS-lane workgroups, as many as stay resident, take work items from one
+// queue (an atomic head counter) until it is empty.  Items (built on the host, see
+// bdx_item in bridct_device.h) are:
+//   SMALL  several whole arrays of one run (small_unit: one HBM read and write per pixel);
+//   ROW    a band of rows of a larger array, input -> output (row_unit);
+//   COL    a strip of columns of that array, in place on the output (col_unit); it waits
+//          until every ROW item of the array has completed;
+//   ROW2   (round trip only) the inverse row band, in place, after every COL item.
+// The queue lists all ROW items of an array before its COL items (and those before its
+// ROW2 items), so a waiting workgroup only waits for items already taken by running
+// workgroups, which never wait on anything later: the schedule cannot deadlock.  The
+// host interleaves the arrays so that the columns of one array run while the rows of the
+// next are in flight, which keeps the intermediate in the caches.  A wait longer than
+// spin_limit sleeps sets the error flag and proceeds instead of hanging.
+#include "bdx_units.h"
+
+#ifndef BDX_MODE
+#error "compile with -DBDX_MODE=<0|1|2>"
+#endif
+
+namespace bdx {
+namespace {
+
+constexpr int TH = BDX_BATCH_THREADS;
+constexpr int ROW_OP = BDX_MODE == 0 ? FA : BDX_MODE == 1 ? IA : FA2;
+
+// Each unit instantiation is its own (non-inlined) function, so that it keeps its own
+// register allocation instead of the union of all paths of the dispatch.
+template <int H, int W>
+__device__ __noinline__ void small_hw(const float *in, float *out, size_t off, int count, float *smem,
+                                         const float *__restrict__ tab)
+{
+    if constexpr (BDX_BATCH_SMALL(H, W)) {
+        static_assert(BDX_BATCH_PER(H, W) >= 1, "a whole array must fit the LDS area");
+        small_unit<H, W, BDX_MODE, TH>(in, out, off, count, smem, tab);
+    }
+}
+
+template <int H>
+__device__ __forceinline__ void small_h(int w, const float *in, float *out, size_t off, int count, float *smem,
+                                        const float *__restrict__ tab)
+{
+    switch (w) {
+    case 8: small_hw<H, 8>(in, out, off, count, smem, tab); break;
+    case 16: small_hw<H, 16>(in, out, off, count, smem, tab); break;
+    case 32: small_hw<H, 32>(in, out, off, count, smem, tab); break;
+    case 64: small_hw<H, 64>(in, out, off, count, smem, tab); break;
+    case 128: small_hw<H, 128>(in, out, off, count, smem, tab); break;
+    case 256: small_hw<H, 256>(in, out, off, count, smem, tab); break;
+    case 512: small_hw<H, 512>(in, out, off, count, smem, tab); break;
+    default: small_hw<H, 1024>(in, out, off, count, smem, tab); break;
+    }
+}
+
+__device__ __forceinline__ void small_any(int h, int w, const float *in, float *out, size_t off, int count,
+                                          float *smem, const float *__restrict__ tab)
+{
+    switch (h) {
+    case 8: small_h<8>(w, in, out, off, count, smem, tab); break;
+    case 16: small_h<16>(w, in, out, off, count, smem, tab); break;
+    case 32: small_h<32>(w, in, out, off, count, smem, tab); break;
+    case 64: small_h<64>(w, in, out, off, count, smem, tab); break;
+    case 128: small_h<128>(w, in, out, off, count, smem, tab); break;
+    case 256: small_h<256>(w, in, out, off, count, smem, tab); break;
+    case 512: small_h<512>(w, in, out, off, count, smem, tab); break;
+    default: small_h<1024>(w, in, out, off, count, smem, tab); break;
+    }
+}
+
+template <int W, int OP>
+__device__ __noinline__ void row_w(const float *in, float *out, size_t row0, int nrows, float *smem,
+                                   const float *__restrict__ tab)
+{
+    static_assert(row_unit_lds<W, OP, TH>() <= BDX_BATCH_LDS && row_unit_rows<W, TH>() == BDX_ROW_UNIT(W),
+                  "row unit geometry");
+    row_unit<W, OP, TH>(in, out, row0, nrows, smem, tab, false);
+}
+
+template <int OP>
+__device__ __forceinline__ void row_any(int w, const float *in, float *out, size_t row0, int nrows, float *smem,
+                                        const float *__restrict__ tab)
+{
+    switch (w) {
+    case 8: row_w<8, OP>(in, out, row0, nrows, smem, tab); break;
+    case 16: row_w<16, OP>(in, out, row0, nrows, smem, tab); break;
+    case 32: row_w<32, OP>(in, out, row0, nrows, smem, tab); break;
+    case 64: row_w<64, OP>(in, out, row0, nrows, smem, tab); break;
+    case 128: row_w<128, OP>(in, out, row0, nrows, smem, tab); break;
+    case 256: row_w<256, OP>(in, out, row0, nrows, smem, tab); break;
+    case 512: row_w<512, OP>(in, out, row0, nrows, smem, tab); break;
+    default: row_w<1024, OP>(in, out, row0, nrows, smem, tab); break;
+    }
+}
+
+template <int H>
+__device__ __noinline__ void col_h(const float *in, float *out, int w, size_t base, int c0, int ncols, float *smem,
+                                   const float *__restrict__ tab)
+{
+    static_assert(col_unit_lds<H, TH>() <= BDX_BATCH_LDS && col_unit_cols<H, TH>() == BDX_COL_UNIT(H),
+                  "column unit geometry");
+    col_unit<H, BDX_MODE, TH>(in, out, w, base, c0, ncols, smem, tab, false);
+}
+
+__device__ __forceinline__ void col_any(int h, int w, const float *in, float *out, size_t base, int c0, int ncols,
+                                        float *smem, const float *__restrict__ tab)
+{
+    switch (h) {
+    case 8: col_h<8>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 16: col_h<16>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 32: col_h<32>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 64: col_h<64>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 128: col_h<128>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 256: col_h<256>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 512: col_h<512>(in, out, w, base, c0, ncols, smem, tab); break;
+    default: col_h<1024>(in, out, w, base, c0, ncols, smem, tab); break;
+    }
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's vector memory
+// operations, cache write-backs and invalidations included, have completed.
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Waits until *p >= target (thread 0 spins, the block follows), then acquires in every
+// wave.  Producers and consumers may sit on different XCDs, whose L2 caches are not
+// coherent with each other: the agent-scope acquire invalidates the stale lines.
+__device__ __forceinline__ void wait_for(unsigned *p, unsigned target, unsigned *err, unsigned limit)
+{
+    if (threadIdx.x == 0) {
+        unsigned n = 0;
+        while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++n > limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    wait_vm();
+}
+
+// Publishes the block's global stores at agent scope, then counts the item as done.  Each
+// wave releases its own stores (waits for them and writes its XCD's L2 back) and waits for
+// the write-back before the barrier; only then does thread 0 count the item.
+__device__ __forceinline__ void signal_done(unsigned *p)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wait_vm();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, const bdx_run *__restrict__ runs,
+                                              const bdx_item *__restrict__ items, unsigned nitems,
+                                              unsigned *ctr, const float *__restrict__ tab, unsigned spin_limit)
+{
+    __shared__ float smem[BDX_BATCH_LDS];
+    __shared__ unsigned s_item;
+    unsigned *err = ctr + BDX_CTR_ERROR;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_item = __hip_atomic_fetch_add(ctr + BDX_CTR_HEAD, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const unsigned i = s_item;
+        __syncthreads();
+        if (i >= nitems)
+            break;
+        const bdx_item it = items[i];
+        const bdx_run rn = runs[it.run];
+        const unsigned kind = it.kind_ctr >> 28, c = it.kind_ctr & 0x0FFFFFFFu;
+        const int h = (int)rn.h, w = (int)rn.w;
+        const size_t base = rn.off + (size_t)it.idx * h * w;
+        unsigned *rows_done = ctr + BDX_CTR_BASE + 2 * c, *cols_done = rows_done + 1;
+        if (kind == BDX_ITEM_SMALL) {
+            small_any(h, w, in, out, base, (int)it.unit, smem, tab);
+        } else if (kind == BDX_ITEM_ROW) {
+            const int r0 = (int)it.unit * BDX_ROW_UNIT(w);
+            row_any<ROW_OP>(w, in + base, out + base, (size_t)r0, min(BDX_ROW_UNIT(w), h - r0), smem, tab);
+            signal_done(rows_done);
+        } else if (kind == BDX_ITEM_COL) {
+            wait_for(rows_done, (unsigned)BDX_CEIL(h, BDX_ROW_UNIT(w)), err, spin_limit);
+            const int c0 = (int)it.unit * BDX_COL_UNIT(h);
+            col_any(h, w, out, out, base, c0, min(BDX_COL_UNIT(h), w - c0), smem, tab);
+            if (BDX_MODE == 2)
+                signal_done(cols_done);
+        } else if constexpr (BDX_MODE == 2) {
+            wait_for(cols_done, (unsigned)BDX_CEIL(w, BDX_COL_UNIT(h)), err, spin_limit);
+            const int r0 = (int)it.unit * BDX_ROW_UNIT(w);
+            row_any<I0>(w, out + base, out + base, (size_t)r0, min(BDX_ROW_UNIT(w), h - r0), smem, tab);
+        }
+        __syncthreads();   // LDS is reused by the next item
+    }
+}
+
+}  // namespace
+
+#define BDX_CAT2(a, b) a##b
+#define BDX_CAT(a, b) BDX_CAT2(a, b)
+
+int BDX_CAT(launch_batch_m, BDX_MODE)(const float *in, float *out, const bdx_run *runs, const bdx_item *items,
+                                      unsigned nitems, unsigned *ctr, const float *tab, unsigned blocks,
+                                      unsigned spin_limit, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, nitems, ctr, tab, spin_limit);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace bdx

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "bridct_device.h"
+
 namespace bdx {
 
 // Line operations: FA forward * alpha, FA2 forward * alpha^2, IA alpha * inverse, I0 inverse.
@@ -38,5 +40,13 @@ int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const 
 // 1 alpha*inverse, 2 forward*alpha^2 then inverse.
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
                 hipStream_t s);
+
+// Persistent batch kernel, one per mode (bdx_batch.hip).
+int launch_batch_m0(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
+int launch_batch_m1(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
+int launch_batch_m2(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
 
 }  // namespace bdx

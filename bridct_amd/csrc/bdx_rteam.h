@@ -26,7 +26,9 @@ template <int T> __device__ __forceinline__ float team_mirror(float x)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xF, 0xF, false));
 }
 
-// LDS copy of the constants of axis length n used by a row team.
+// LDS copy of the constants of axis length n used by a team; XCH is the per-team LDS
+// region of the forward (FFT exchange, then the output row) and XCHI of the inverse
+// (exchange only), padded so that the teams of one wave start on different banks.
 template <int n> struct RowTab {
     static constexpr int N = n / 2;
     static constexpr int TW = 0;                // team twiddles, 2 (k1 T + t)
@@ -34,9 +36,8 @@ template <int n> struct RowTab {
     static constexpr int TN = 2 * n;            // tan(pi k / (2n)), k <= N
     static constexpr int AL = 2 * n + N + 4;    // normalization (alpha or alpha^2)
     static constexpr int SIZE = AL + n;
-    // per-team LDS: the FFT exchange (N floats) and then the output row (n floats), padded
-    // so that the teams of one wave start on different banks
     static constexpr int XCH = n + BDX_TEAM(n);
+    static constexpr int XCHI = N + BDX_TEAM(n);
 };
 
 template <int n, int OP, int THREADS>

@@ -31,6 +31,12 @@ int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int 
     }
 }
 
+__global__ void k_zero(unsigned *p, unsigned n)
+{
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -91,6 +97,68 @@ int bdx_copy_d2h(void *dst, const void *src, size_t bytes, void *stream)
 int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream)
 {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_device_cus(void)
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    return n;
+}
+
+int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
+                     unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    // a fill kernel, not a memset node, so that the sequence can be captured in a graph
+    hipLaunchKernelGGL(k_zero, dim3((ncounters + 255) / 256), dim3(256), 0, s, d_counters, ncounters);
+    if (hipGetLastError() != hipSuccess || nitems == 0)
+        return nitems == 0 ? 0 : -1;
+    switch (mode) {
+    case BDX_FORWARD:
+        return bdx::launch_batch_m0(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+    case BDX_INVERSE:
+        return bdx::launch_batch_m1(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+    case BDX_ROUNDTRIP:
+        return bdx::launch_batch_m2(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+    default:
+        return -1;
+    }
+}
+
+int bdx_graph_capture(void *stream, int (*fn)(void *), void *arg, void **graph_exec)
+{
+    hipStream_t s = (hipStream_t)stream;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return -1;
+    const int rc = fn(arg);
+    const hipError_t end = hipStreamEndCapture(s, &graph);
+    if (rc || end != hipSuccess || !graph) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return -1;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess)
+        return -1;
+    *graph_exec = (void *)exec;
+    return 0;
+}
+
+int bdx_graph_launch(void *graph_exec, void *stream)
+{
+    return hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_graph_destroy(void *graph_exec)
+{
+    return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? 0 : -1;
 }
 
 int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,

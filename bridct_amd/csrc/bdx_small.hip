@@ -3,8 +3,7 @@
 // trip), store -- one HBM read and one HBM write per pixel.  Several small arrays share a
 // workgroup.  This file is compiled once per array height BDX_H; each (BDX_H, w) pair is
 // its own kernel, so strides and array geometry are compile-time constants.
-#include "bdx_kernels.h"
-#include "bdx_lines.h"
+#include "bdx_units.h"
 
 #ifndef BDX_H
 #error "compile with -DBDX_H=<height>"
@@ -18,36 +17,9 @@ __global__ __launch_bounds__(kThreads) void k_small(const float *in, float *out,
                                                      const float *__restrict__ tab)
 {
     extern __shared__ float tile[];
-    constexpr int P = W + 1;
     const size_t a0 = (size_t)blockIdx.x * per;
     const int na = (int)min((size_t)per, batch - a0);
-    const size_t off = a0 * H * W;
-    const auto row_base = [](int L) { return L * P; };
-    const auto col_base = [](int L) {
-        const int a = L / W, c = L - a * W;
-        return a * H * P + c;
-    };
-    tile_load<W, P, kThreads>(tile, in + off, W, na * H);
-    __syncthreads();
-    if constexpr (MODE == 0) {
-        run_lines<W, 1, FA, kThreads>(tile, na * H, row_base, tab);
-        __syncthreads();
-        run_lines<H, P, FA, kThreads>(tile, na * W, col_base, tab);
-    } else if constexpr (MODE == 1) {
-        run_lines<W, 1, IA, kThreads>(tile, na * H, row_base, tab);
-        __syncthreads();
-        run_lines<H, P, IA, kThreads>(tile, na * W, col_base, tab);
-    } else {
-        run_lines<W, 1, FA2, kThreads>(tile, na * H, row_base, tab);
-        __syncthreads();
-        run_lines<H, P, FA2, kThreads>(tile, na * W, col_base, tab);
-        __syncthreads();
-        run_lines<H, P, I0, kThreads>(tile, na * W, col_base, tab);
-        __syncthreads();
-        run_lines<W, 1, I0, kThreads>(tile, na * H, row_base, tab);
-    }
-    __syncthreads();
-    tile_store<W, P, kThreads>(tile, out + off, W, na * H);
+    small_unit<H, W, MODE, kThreads>(in, out, a0 * H * W, na, tile, tab);
 }
 
 template <int H, int W>

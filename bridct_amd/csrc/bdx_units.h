@@ -1,0 +1,201 @@
+// Block-level work units of the 2D transform, shared by the per-shape kernels
+// (bdx_small.hip, bdx_passes.hip) and the persistent batch kernel (bdx_batch.hip).
+//
+// Every unit runs on one workgroup of THREADS lanes, uses the LDS area `smem` it is given
+// (at most unit_lds<...>() floats) and touches only its own part of the output, so units
+// of the same pass can run in any order and concurrently.
+//   small_unit   count whole h x w arrays: load, rows, columns (and the inverse halves of
+//                the fused round trip), store; one HBM read and write per pixel.
+//   row_unit     a band of full rows of a two-pass array (row pass input -> output).
+//   col_unit     a strip of full columns of a two-pass array, in place on the output.
+// Reference equivalents: src/rect.c:5-23 (column pass, banded row pass, normalization),
+// src/generated/small_optimized.c (whole small arrays).
+#pragma once
+#include "bdx_kernels.h"
+#include "bdx_lines.h"
+#include "bdx_rteam.h"
+#include "bdx_wteam.h"
+
+namespace bdx {
+
+// ---- whole arrays --------------------------------------------------------------------
+
+template <int H, int W, int THREADS> constexpr int small_per_unit(int lds_floats) { return lds_floats / (H * (W + 1)); }
+
+template <int H, int W, int MODE, int THREADS>
+__device__ __forceinline__ void small_unit(const float *in, float *out, size_t off, int count, float *tile,
+                                           const float *__restrict__ tab)
+{
+    constexpr int P = W + 1;
+    const auto row_base = [](int L) { return L * P; };
+    const auto col_base = [](int L) {
+        const int a = L / W, c = L - a * W;
+        return a * H * P + c;
+    };
+    tile_load<W, P, THREADS>(tile, in + off, W, count * H);
+    __syncthreads();
+    if constexpr (MODE == 0) {
+        run_lines<W, 1, FA, THREADS>(tile, count * H, row_base, tab);
+        __syncthreads();
+        run_lines<H, P, FA, THREADS>(tile, count * W, col_base, tab);
+    } else if constexpr (MODE == 1) {
+        run_lines<W, 1, IA, THREADS>(tile, count * H, row_base, tab);
+        __syncthreads();
+        run_lines<H, P, IA, THREADS>(tile, count * W, col_base, tab);
+    } else {
+        run_lines<W, 1, FA2, THREADS>(tile, count * H, row_base, tab);
+        __syncthreads();
+        run_lines<H, P, FA2, THREADS>(tile, count * W, col_base, tab);
+        __syncthreads();
+        run_lines<H, P, I0, THREADS>(tile, count * W, col_base, tab);
+        __syncthreads();
+        run_lines<W, 1, I0, THREADS>(tile, count * H, row_base, tab);
+    }
+    __syncthreads();
+    tile_store<W, P, THREADS>(tile, out + off, W, count * H);
+}
+
+// ---- row pass ------------------------------------------------------------------------
+
+// Rows per unit: one team per row for the team path, one lane per row otherwise.
+template <int W, int THREADS> constexpr int row_unit_rows() { return W >= 128 ? THREADS / team_size<W>() : THREADS; }
+
+template <int W, int OP, int THREADS> constexpr int row_unit_lds()
+{
+    if constexpr (W >= 128)
+        return RowTab<W>::SIZE + row_unit_rows<W, THREADS>() * (OP == FA || OP == FA2 ? RowTab<W>::XCH : RowTab<W>::XCHI);
+    else
+        return THREADS * (W + 1);
+}
+
+// Rows [row0, row0 + nrows) of the row sequence (all rows of all arrays of the same width
+// are one sequence), nrows <= row_unit_rows.  With `tables_ready` the LDS constant table
+// of width W is already in smem.
+template <int W, int OP, int THREADS>
+__device__ __forceinline__ void row_unit(const float *in, float *out, size_t row0, int nrows, float *smem,
+                                         const float *__restrict__ tab, bool tables_ready)
+{
+    if constexpr (W >= 128) {
+        using R = RowTab<W>;
+        constexpr int T = team_size<W>();
+        constexpr int X = OP == FA || OP == FA2 ? R::XCH : R::XCHI;
+        float *lt = smem, *xl = smem + R::SIZE;
+        if (!tables_ready) {
+            load_row_tables<W, OP, THREADS>(lt, tab);
+            __syncthreads();
+        }
+        const int team = threadIdx.x / T, t = threadIdx.x % T;
+        const size_t row = row0 + team;
+        if (team < nrows) {   // a team works alone on its row: no block barrier follows
+            if constexpr (OP == FA || OP == FA2)
+                rteam_fwd<W, OP>(in + row * W, out + row * W, t, xl + team * X, lt);
+            else
+                rteam_inv<W, OP>(in + row * W, out + row * W, t, xl + team * X, lt);
+        }
+    } else {
+        constexpr int P = W + 1;
+        tile_load<W, P, THREADS>(smem, in + row0 * W, W, nrows);
+        __syncthreads();
+        run_lines<W, 1, OP, THREADS>(smem, nrows, [](int L) { return L * P; }, tab);
+        __syncthreads();
+        tile_store<W, P, THREADS>(smem, out + row0 * W, W, nrows);
+    }
+}
+
+// ---- column pass ---------------------------------------------------------------------
+
+template <int H, int THREADS> constexpr int col_unit_cols()
+{
+    if constexpr (H >= 128)
+        return CTeam<H, THREADS>::L;
+    else
+        return THREADS;
+}
+
+template <int H, int THREADS> constexpr int col_unit_lds()
+{
+    if constexpr (H >= 128)
+        return CTeam<H, THREADS>::XLDS + CTeam<H, THREADS>::TLDS;
+    else
+        return 1;
+}
+
+// Columns [c0, c0 + ncols) of the h x w array at `base` (in and out may alias),
+// ncols <= col_unit_cols; lanes past the strip compute on zeros and store nothing.
+// MODE 0 forward * alpha, 1 alpha * inverse, 2 forward * alpha^2 then inverse.
+template <int H, int MODE, int THREADS>
+__device__ __forceinline__ void col_unit(const float *in, float *out, int w, size_t base, int c0, int ncols,
+                                         float *smem, const float *__restrict__ tab, bool tables_ready)
+{
+    if constexpr (H >= 128) {
+        using C = CTeam<H, THREADS>;
+        constexpr int T = C::T, Q = C::Q, L = C::L, M = C::M;
+        float *xl = smem, *lt = smem + C::XLDS;
+        if (M > 1 && !tables_ready) {
+            load_row_tables<H, MODE == 2 ? FA2 : FA, THREADS>(lt, tab);
+            __syncthreads();
+        }
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int t = M == 1 ? __builtin_amdgcn_readfirstlane(wave) : wave * M + lane / L;
+        const int line = lane % L;
+        const float *src = in + base + c0;
+        float *dst = out + base + c0;
+        const bool live = line < ncols;
+        const auto ld = [&](int i) { return live ? src[(unsigned)(i * w) + line] : 0.0f; };
+        const auto st = [&](int i, float v) {
+            if (live)
+                dst[(unsigned)(i * w) + line] = v;
+        };
+        const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
+        const CConst<H, M> cc{tab + BDX_TABLE_BASE(H), lt};
+        float o[Q][T][4];
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int r = 0; r < Q; r++)
+#pragma unroll
+                for (int k2 = 0; k2 < T; k2++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int idx = wteam_index<H, T>(t + T * r, k2, c);
+                        o[r][k2][c] = ld(idx) * al[idx];
+                    }
+            wteam_inv<H, THREADS>(t, line, o, st, xl, cc);
+        } else {
+            wteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int r, int k2, int c, float v) {
+                const int idx = wteam_index<H, T>(t + T * r, k2, c);
+                if constexpr (MODE == 0)
+                    st(idx, v * al[idx]);
+                else
+                    o[r][k2][c] = v * al[idx];
+            });
+            if constexpr (MODE == 2)
+                wteam_inv<H, THREADS>(t, line, o, st, xl, cc);
+        }
+    } else {
+        // one lane per column: the generated Shao-Johnson core on H values in registers
+        if ((int)threadIdx.x >= ncols)
+            return;
+        const size_t off = base + c0 + threadIdx.x;
+        const float *al = alpha_of<H>(tab, 1);
+        const float *al2 = alpha_of<H>(tab, 2);
+        float x[H];
+#pragma unroll
+        for (int i = 0; i < H; i++)
+            x[i] = in[off + (size_t)i * w] * (MODE == 1 ? al[i] : 1.0f);
+        if constexpr (MODE == 1) {
+            Sj<H>::inv(x);
+        } else {
+            Sj<H>::fwd(x);
+#pragma unroll
+            for (int i = 0; i < H; i++)
+                x[i] *= MODE == 0 ? al[i] : al2[i];
+            if constexpr (MODE == 2)
+                Sj<H>::inv(x);
+        }
+#pragma unroll
+        for (int i = 0; i < H; i++)
+            out[off + (size_t)i * w] = x[i];
+    }
+}
+
+}  // namespace bdx

@@ -1,23 +1,35 @@
-// Column-style team lines: lane = line, team member = wave.
+// Column-style team lines: lane = line, team member = (part of) a wave.
 //
-// A block of T waves (T = team_size<n>()) transforms 64 lines of length n at once, one per
-// lane; wave t is team member t for every lane.  All twiddles, tangents, normalization
-// factors and element indices depend only on (t, compile-time slot), so they are
-// wave-uniform (scalar loads, no per-lane selects), and the loads and stores of element
-// i of the 64 lines are one coalesced 256-byte access when the lines are adjacent
-// columns.  The exchange between members goes through LDS with block barriers, in
-// ROUNDS slices of 64 / ROUNDS lanes to bound the LDS footprint to n * 64 / ROUNDS floats.
-// Algorithm and index maps as in bdx_lines.h (team of T lanes per line) and bdx_post.h.
+// A block transforms L = 64 / M lines of length n, one per lane, with the T members of
+// each line's team spread over T / M waves (M members per wave: lanes [0, L) of a wave
+// are member M w, lanes [L, 2L) member M w + 1).  With M = 1 every twiddle, tangent,
+// normalization factor and element index is wave-uniform (scalar loads, no selects);
+// with M = 2 (used for n = 1024, where one member per wave would need 1024-lane blocks
+// and leave one block per CU) the constants come from an LDS copy of the table block
+// and two half-wave values per wave.  Loads and stores of element i of adjacent columns
+// are 256-byte (M = 1) or 128-byte (M = 2) row segments.  The exchange between members
+// goes through LDS with block barriers, in ROUNDS slices of the lines to bound the LDS
+// footprint.  Algorithm and index maps as in bdx_lines.h and bdx_post.h.
 #pragma once
 #include "bdx_lines.h"
 #include "bdx_post.h"
+#include "bdx_rteam.h"
 
 namespace bdx {
 
-template <int n> constexpr int wteam_rounds() { return n * 64 > 32768 ? 2 : 1; }
-template <int n> constexpr int wteam_lds() { return n * 64 / wteam_rounds<n>(); }
+// Geometry of a column team of length n run by blocks of THREADS lanes.
+template <int n, int THREADS> struct CTeam {
+    static constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
+    static constexpr int M = 64 * T / THREADS;            // members per wave
+    static constexpr int L = 64 / M;                      // lines per block
+    static constexpr int ROUNDS = n * L > 16384 ? 2 : 1;  // keep the exchange at 64 KiB or less
+    static constexpr int SL = L / ROUNDS;                 // lines per exchange round
+    static constexpr int XLDS = n * SL;                   // exchange floats
+    static constexpr int TLDS = M > 1 ? RowTab<n>::SIZE : 1;   // LDS constant table
+    static_assert(M >= 1 && 64 % M == 0, "team members must tile the wave");
+};
 
-// Element index of output slot (k2, which) of pair group g (wave-uniform g).
+// Element index of output slot (k2, which) of pair group g.
 template <int n, int T>
 __device__ __forceinline__ int wteam_index(int g, int k2, int which)
 {
@@ -33,16 +45,33 @@ __device__ __forceinline__ int wteam_index(int g, int k2, int which)
     return which == 0 ? k : which == 1 ? n - k : which == 2 ? N - k : N + k;
 }
 
+// Constants of member t: from the global table block (M = 1, wave-uniform) or from the
+// LDS copy (M > 1).
+template <int n, int M> struct CConst {
+    static constexpr int T = team_size<n>();
+    const float *g, *l;
+    __device__ __forceinline__ float tw_re(int k1, int t) const
+    {
+        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1))] : l[RowTab<n>::TW + 2 * (k1 * T + t)];
+    }
+    __device__ __forceinline__ float tw_im(int k1, int t) const
+    {
+        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1)) + 1] : l[RowTab<n>::TW + 2 * (k1 * T + t) + 1];
+    }
+    __device__ __forceinline__ float om_re(int k) const { return M == 1 ? g[2 * k] : l[RowTab<n>::OM + 2 * k]; }
+    __device__ __forceinline__ float om_im(int k) const { return M == 1 ? g[2 * k + 1] : l[RowTab<n>::OM + 2 * k + 1]; }
+    __device__ __forceinline__ float tn(int k) const { return M == 1 ? g[2 * n + k] : l[RowTab<n>::TN + k]; }
+};
+
 // Forward core.  ld(i) returns element i of the lane's line; emit(r, k2, w, v) receives
 // the unscaled output of index wteam_index(t + T r, k2, w) as soon as it is computed.
-template <int n, typename Ld, typename Emit>
-__device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *__restrict__ tab, Emit emit)
+template <int n, int THREADS, typename Ld, typename Emit>
+__device__ __forceinline__ void wteam_fwd(int t, int line, Ld ld, float *xl, CConst<n, CTeam<n, THREADS>::M> cc,
+                                          Emit emit)
 {
-    constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
-    constexpr int ROUNDS = wteam_rounds<n>(), SL = 64 / ROUNDS;
-    const int lane = threadIdx.x & 63, sl = lane % SL, rnd = lane / SL;
-    const float *om = tab + BDX_TABLE_BASE(n);
-    const float *tn = om + 2 * n;
+    using C = CTeam<n, THREADS>;
+    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
+    const int sl = line % SL, rnd = line / SL;
     float re[N1], im[N1];
 #pragma unroll
     for (int j1 = 0; j1 < N1; j1++) {
@@ -52,10 +81,10 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
         im[j1] = ld(i1);
     }
     Dft<N1, -1>::run(re, im);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k1 = 1; k1 < N1; k1++) {
-        const int e = (2 * t * k1) & (n - 1);
-        const float wr = om[2 * e], wi = om[2 * e + 1];
+        const float wr = cc.tw_re(k1, t), wi = cc.tw_im(k1, t);
         const float a = re[k1], b = im[k1];
         re[k1] = __builtin_fmaf(a, wr, -b * wi);
         im[k1] = __builtin_fmaf(a, wi, b * wr);
@@ -88,6 +117,7 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
         }
         __syncthreads();
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < Q; r++) {
         const int g = t + T * r;
@@ -99,7 +129,7 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
                 float o[4];
                 if (k2 == T / 2) {
                     post_fwd_special(dr[r][0][0], di[r][0][0], dr[r][0][T / 2], di[r][0][T / 2],
-                                     tn[n / 4], o);
+                                     cc.tn(n / 4), o);
                 } else {
                     const bool lo = k2 < T / 2;
                     const int j = k2 - T / 2;
@@ -108,7 +138,7 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
                     const float ai = lo ? di[r][1][k2] : di[r][0][j];
                     const float br = lo ? dr[r][1][T - 1 - k2] : dr[r][0][T - j];
                     const float bi = lo ? di[r][1][T - 1 - k2] : di[r][0][T - j];
-                    post_fwd(ar, ai, br, bi, om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o);
+                    post_fwd(ar, ai, br, bi, cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), o);
                 }
 #pragma unroll
                 for (int w = 0; w < 4; w++)
@@ -120,7 +150,7 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
                 const int k = g + N1 * k2;
                 float o[4];
                 post_fwd(dr[r][0][k2], di[r][0][k2], dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2],
-                         om[2 * k], om[2 * k + 1], tn[k], tn[N - k], o);
+                         cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), o);
 #pragma unroll
                 for (int w = 0; w < 4; w++)
                     emit(r, k2, w, o[w]);
@@ -131,15 +161,13 @@ __device__ __forceinline__ void wteam_fwd(int t, Ld ld, float *xl, const float *
 
 // Inverse core: o[r][k2][w] holds the (already scaled) input of index wteam_index(...);
 // st(i, v) stores element i of the lane's line.
-template <int n, typename St>
-__device__ __forceinline__ void wteam_inv(int t, float (&o)[n / 2 / team_size<n>() / (2 * team_size<n>())][team_size<n>()][4],
-                                          St st, float *xl, const float *__restrict__ tab)
+template <int n, int THREADS, typename St>
+__device__ __forceinline__ void wteam_inv(int t, int line, float (&o)[CTeam<n, THREADS>::Q][CTeam<n, THREADS>::T][4],
+                                          St st, float *xl, CConst<n, CTeam<n, THREADS>::M> cc)
 {
-    constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
-    constexpr int ROUNDS = wteam_rounds<n>(), SL = 64 / ROUNDS;
-    const int lane = threadIdx.x & 63, sl = lane % SL, rnd = lane / SL;
-    const float *om = tab + BDX_TABLE_BASE(n);
-    const float *tn = om + 2 * n;
+    using C = CTeam<n, THREADS>;
+    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
+    const int sl = line % SL, rnd = line / SL;
     float dr[Q][2][T], di[Q][2][T];
 #pragma unroll
     for (int r = 0; r < Q; r++) {
@@ -148,7 +176,7 @@ __device__ __forceinline__ void wteam_inv(int t, float (&o)[n / 2 / team_size<n>
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 if (k2 == T / 2) {
-                    post_inv_special(o[r][k2], tn[n / 4], dr[r][0][0], di[r][0][0], dr[r][0][T / 2],
+                    post_inv_special(o[r][k2], cc.tn(n / 4), dr[r][0][0], di[r][0][0], dr[r][0][T / 2],
                                      di[r][0][T / 2]);
                     continue;
                 }
@@ -156,7 +184,7 @@ __device__ __forceinline__ void wteam_inv(int t, float (&o)[n / 2 / team_size<n>
                 const int j = k2 - T / 2;
                 const int k = lo ? N1 / 2 + N1 * k2 : N1 * j;
                 float ar, ai, br, bi;
-                post_inv(o[r][k2], om[2 * k], om[2 * k + 1], tn[k], tn[N - k], ar, ai, br, bi);
+                post_inv(o[r][k2], cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), ar, ai, br, bi);
                 if (lo) {
                     dr[r][1][k2] = ar;
                     di[r][1][k2] = ai;
@@ -173,8 +201,8 @@ __device__ __forceinline__ void wteam_inv(int t, float (&o)[n / 2 / team_size<n>
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 const int k = g + N1 * k2;
-                post_inv(o[r][k2], om[2 * k], om[2 * k + 1], tn[k], tn[N - k], dr[r][0][k2], di[r][0][k2],
-                         dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2]);
+                post_inv(o[r][k2], cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), dr[r][0][k2],
+                         di[r][0][k2], dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2]);
             }
         }
         Dft<T, 1>::run(dr[r][0], di[r][0]);
@@ -210,8 +238,7 @@ __device__ __forceinline__ void wteam_inv(int t, float (&o)[n / 2 / team_size<n>
     }
 #pragma unroll
     for (int k1 = 1; k1 < N1; k1++) {
-        const int e = (2 * t * k1) & (n - 1);
-        const float wr = om[2 * e], wi = om[2 * e + 1];
+        const float wr = cc.tw_re(k1, t), wi = cc.tw_im(k1, t);
         const float a = re[k1], b = im[k1];
         re[k1] = __builtin_fmaf(a, wr, b * wi);
         im[k1] = __builtin_fmaf(b, wr, -a * wi);

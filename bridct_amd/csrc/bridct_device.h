@@ -20,6 +20,71 @@ extern "C" {
 /* Team size of the multi-lane 1D transform of length n (1 below 128). */
 #define BDX_TEAM(n) ((n) <= 64 ? 1 : (n) >= 1024 ? 16 : (n) >= 256 ? 8 : 4)
 
+/* ---- persistent batch kernel (shared geometry, host C and device) ----
+ * A batch is a sequence of runs; a run is `count` contiguous h x w arrays at element
+ * offset `off` of the batch buffers.  Arrays with both axes at most 128 are transformed
+ * whole (several per unit); the others by row units of up to BDX_ROW_UNIT(w) rows and
+ * column units of up to BDX_COL_UNIT(h) columns (256-lane workgroups; the last unit of
+ * an axis may be partial). */
+#define BDX_BATCH_THREADS 256
+#define BDX_BATCH_LDS 20400   /* floats: two workgroups fit the 160 KiB LDS of a CU */
+#define BDX_BATCH_SMALL(h, w) ((h) <= 128 && (w) <= 128 && (long)(h) * ((w) + 1) <= BDX_BATCH_LDS)
+#define BDX_BATCH_PER(h, w) (BDX_BATCH_LDS / ((h) * ((w) + 1)))
+#define BDX_ROW_UNIT(w) ((w) >= 128 ? BDX_BATCH_THREADS / BDX_TEAM(w) : BDX_BATCH_THREADS)
+#define BDX_COL_UNIT(h) ((h) >= 128 ? BDX_BATCH_THREADS / BDX_TEAM(h) : BDX_BATCH_THREADS)
+#define BDX_CEIL(a, b) (((a) + (b) - 1) / (b))
+/* work item kinds */
+#define BDX_ITEM_SMALL 0u
+#define BDX_ITEM_ROW 1u
+#define BDX_ITEM_COL 2u
+#define BDX_ITEM_ROW2 3u
+/* counters block: [0] queue head, [1] error flag, then per two-pass array
+ * [2 + 2 c] row units done, [3 + 2 c] column units done */
+#define BDX_CTR_HEAD 0
+#define BDX_CTR_ERROR 1
+#define BDX_CTR_BASE 2
+
+typedef struct {
+    unsigned h, w, count, pad;
+    unsigned long long off;
+} bdx_run;
+
+/* kind_ctr = kind << 28 | two-pass array counter id (row/column kinds).
+ * small: idx = first array of the run, unit = number of arrays.
+ * row/column: idx = array within the run, unit = unit index. */
+typedef struct {
+    unsigned run, idx, unit, kind_ctr;
+} bdx_item;
+
+/* Launches the persistent batch kernel (after a counter reset) on `stream`:
+ * `blocks` resident workgroups pull `nitems` items.  spin_limit bounds each dependency
+ * wait (a wait that runs out sets the error flag instead of hanging). */
+int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
+                     unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, void *stream);
+int bdx_device_cus(void);
+
+/* Host-side schedule of a batch (no device needed): runs of consecutive equal shapes,
+ * the item queues ([0] forward / inverse, [1] fused round trip) and the counter count.
+ * `lag` = queue distance between an array's last row item and its first column item.
+ * Returns 0, or -1 for an invalid shape list or allocation failure. */
+typedef struct {
+    size_t count, elements;
+    size_t *offsets;
+    bdx_run *runs;
+    unsigned nruns, ntwo;
+    bdx_item *items[2];
+    unsigned nitems[2];
+    unsigned nctr;
+} bdx_schedule;
+int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s);
+void bdx_schedule_free(bdx_schedule *s);
+
+/* Graph capture of a sequence of calls issued by `fn(arg)` on `stream`. */
+int bdx_graph_capture(void *stream, int (*fn)(void *), void *arg, void **graph_exec);
+int bdx_graph_launch(void *graph_exec, void *stream);
+int bdx_graph_destroy(void *graph_exec);
+
 /* Transform modes seen by the kernels (same values as enum bridct_mode). */
 #define BDX_FORWARD 0
 #define BDX_INVERSE 1

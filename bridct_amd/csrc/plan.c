@@ -15,6 +15,7 @@
 
 #include "bridct_device.h"
 #include "generated/sj_factors.h"
+#include "host_internal.h"
 
 #include <math.h>
 #include <stdint.h>
@@ -79,7 +80,7 @@ static double axis_alpha(int n, int k)
     return q * cos(PI * kk / (2.0 * n)) * (special ? 1.0 : 0.5);
 }
 
-static void build_tables(float *t)
+void bridct_build_tables(float *t)
 {
     memset(t, 0, BDX_TABLE_FLOATS * sizeof(float));
     for (int n = 8; n <= 1024; n *= 2) {
@@ -140,7 +141,7 @@ bridct_plan *bridct_plan_create(int height, int width)
     float *host = malloc(BDX_TABLE_FLOATS * sizeof(float));
     if (!host)
         goto fail;
-    build_tables(host);
+    bridct_build_tables(host);
     const size_t stage = plan->chunk * plan->elements * sizeof(float);
     if (bdx_stream_create(&plan->stream) ||
         bdx_malloc((void **)&plan->d_tables, BDX_TABLE_FLOATS * sizeof(float)) ||

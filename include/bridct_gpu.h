@@ -43,6 +43,39 @@ BRIDCT_API int bridct_plan_synchronize(bridct_plan *plan);
 /* The plan's stream (a hipStream_t), for callers that order their own work after it. */
 BRIDCT_API void *bridct_plan_stream(const bridct_plan *plan);
 
+/* ---- ragged batches ----
+ * No counterpart in the reference, whose callers loop over arrays with one plan per
+ * shape (src/plan.c:126-163, python/bridct_numpy.c:96-112).  A batch describes `count`
+ * arrays of shapes heights[i] x widths[i] stored back to back in that order (array i
+ * starts at element bridct_batch_offset(batch, i)); one call transforms all of them with
+ * one persistent kernel launch (two for BRIDCT_GPU_FORWARD_THEN_INVERSE).  Each array's
+ * result is that of a plan of its shape.  A batch object, like a plan, must not be used
+ * from two threads at once. */
+typedef struct bridct_batch bridct_batch;
+
+/* NULL when count is 0, a shape is unsupported, no GPU is usable, or allocation fails. */
+BRIDCT_API bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *widths);
+BRIDCT_API void bridct_batch_destroy(bridct_batch *batch);   /* NULL is a no-op */
+/* Total floats of one batch buffer; element offset of array `index` (0 if out of range). */
+BRIDCT_API size_t bridct_batch_elements(const bridct_batch *batch);
+BRIDCT_API size_t bridct_batch_offset(const bridct_batch *batch, size_t index);
+/* Work items one call of `mode` queues (diagnostics). */
+BRIDCT_API size_t bridct_batch_items(const bridct_batch *batch, int mode);
+/* Device buffers, asynchronous on the batch stream.  mode: FORWARD, INVERSE, ROUNDTRIP or
+ * BRIDCT_GPU_FORWARD_THEN_INVERSE.  Buffers 16-byte aligned, identical or disjoint. */
+BRIDCT_API int bridct_batch_apply_device(bridct_batch *batch, int mode, const float *d_input,
+                                         float *d_output);
+/* Same work replayed from a captured HIP graph; the graph is captured on the first call and
+ * again whenever (mode, d_input, d_output) changes. */
+BRIDCT_API int bridct_batch_graph_apply(bridct_batch *batch, int mode, const float *d_input,
+                                        float *d_output);
+/* Waits for the batch stream; BRIDCT_DEVICE_ERROR if a call since the last one failed on
+ * the device (a dependency wait of the persistent kernel that ran out). */
+BRIDCT_API int bridct_batch_synchronize(bridct_batch *batch);
+/* Host buffers of bridct_batch_elements floats each (disjoint); synchronous. */
+BRIDCT_API int bridct_batch_apply(bridct_batch *batch, int mode, const float *input, float *output);
+BRIDCT_API void *bridct_batch_stream(const bridct_batch *batch);
+
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);
 BRIDCT_API int bridct_gpu_free(void *d_ptr);

@@ -1,0 +1,189 @@
+"""GPU tests of ragged batches (one persistent launch over arrays of mixed shapes): each
+array's result equals the plan path of its shape (reference src/plan.c:126-163 applied
+per array) and the FP64 oracle; graph replays equal eager calls."""
+import numpy as np
+import pytest
+
+import support as S
+from bridct_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+FTI = 3   # BRIDCT_GPU_FORWARD_THEN_INVERSE
+
+
+def ragged(shapes, seed):
+    rng = np.random.default_rng(seed)
+    return [(rng.uniform(-1, 1, (h, w)) * 10 ** rng.uniform(-3, 3)).astype(np.float32) for h, w in shapes]
+
+
+def flat(arrays):
+    return np.concatenate([a.ravel() for a in arrays]).astype(np.float32)
+
+
+def plan_results(shapes, arrays, mode):
+    out = []
+    plans = {}
+    for (h, w), a in zip(shapes, arrays):
+        if (h, w) not in plans:
+            plans[(h, w)] = capi.Plan(h, w)
+        y = np.empty_like(a)
+        assert plans[(h, w)].apply_batch(mode, a, y) == 0
+        out.append(y)
+    for p in plans.values():
+        p.close()
+    return out
+
+
+def run_device(batch, mode, x, graph=False, in_place=False):
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_out = d_in if in_place else capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    fn = batch.graph_apply if graph else batch.apply_device
+    assert fn(mode, d_in.ptr, d_out.ptr) == 0
+    assert batch.synchronize() == 0
+    y = np.empty_like(x)
+    d_out.download(y)
+    return y
+
+
+def all_shapes():
+    return [(h, w) for h in S.LENGTHS for w in S.LENGTHS]
+
+
+@pytest.mark.parametrize('mode', [0, 1, 2, FTI])
+def test_every_shape_once(mode):
+    shapes = all_shapes()
+    arrays = ragged(shapes, 11)
+    b = capi.Batch(shapes)
+    try:
+        print('batch of 64 shapes, mode', mode, flush=True)
+        y = b.split(run_device(b, mode, flat(arrays)))
+        ref = plan_results(shapes, arrays, mode)
+        for (h, w), a, yi, ri in zip(shapes, arrays, y, ref):
+            scale = np.abs(ri).max()
+            assert np.abs(yi - ri).max() <= 1e-6 * scale, (h, w)
+            if mode in (0, 1):
+                o = S.oracle_2d(a, mode)
+                assert S.max_rel(yi, o) < 1e-6, (h, w)
+            else:
+                assert np.abs(yi - a).max() <= 2e-6 * np.abs(a).max(), (h, w)
+    finally:
+        b.close()
+
+
+def test_ragged_seeded_mix():
+    rng = np.random.default_rng(7)
+    shapes = [(int(rng.choice(S.LENGTHS)), int(rng.choice(S.LENGTHS))) for _ in range(400)]
+    arrays = ragged(shapes, 7)
+    b = capi.Batch(shapes)
+    try:
+        x = flat(arrays)
+        for mode in (0, 1, 2):
+            print('ragged 400 arrays, mode', mode, flush=True)
+            y = b.split(run_device(b, mode, x))
+            ref = plan_results(shapes, arrays, mode)
+            for (h, w), yi, ri in zip(shapes, y, ref):
+                assert np.abs(yi - ri).max() <= 1e-6 * np.abs(ri).max(), (h, w)
+        print('ragged 400 arrays, in place', flush=True)
+        y_in_place = run_device(b, 0, x, in_place=True)
+        assert np.array_equal(y_in_place, run_device(b, 0, x))
+    finally:
+        b.close()
+
+
+def test_uniform_small_blocks():
+    """8x8 blocks of pixel values (integers 0..255 minus 128) in one launch."""
+    n = 65536
+    rng = np.random.default_rng(3)
+    x = (rng.integers(0, 256, (n, 8, 8)) - 128).astype(np.float32)
+    b = capi.Batch([(8, 8)] * n)
+    p = capi.Plan(8, 8)
+    try:
+        for mode in (0, 1, 2):
+            print('8x8 blocks, mode', mode, flush=True)
+            y = run_device(b, mode, x.ravel()).reshape(n, 8, 8)
+            r = np.empty_like(x)
+            assert p.apply_batch(mode, x, r) == 0
+            assert np.abs(y - r).max() <= 1e-6 * np.abs(r).max()
+    finally:
+        b.close()
+        p.close()
+
+
+def test_large_arrays_with_small_fillers():
+    shapes = [(1024, 1024)] * 3 + [(16, 16)] * 500 + [(512, 1024), (8, 1024), (1024, 8)]
+    arrays = ragged(shapes, 5)
+    b = capi.Batch(shapes)
+    try:
+        for mode in (2, FTI):
+            print('large + fillers, mode', mode, flush=True)
+            y = b.split(run_device(b, mode, flat(arrays)))
+            for (h, w), a, yi in zip(shapes, arrays, y):
+                assert np.abs(yi - a).max() <= 2e-6 * np.abs(a).max(), (h, w)
+    finally:
+        b.close()
+
+
+def test_graph_replays_match_eager():
+    rng = np.random.default_rng(9)
+    shapes = [(int(rng.choice(S.LENGTHS)), int(rng.choice(S.LENGTHS))) for _ in range(60)]
+    arrays = ragged(shapes, 9)
+    x = flat(arrays)
+    b = capi.Batch(shapes)
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_out = capi.DeviceBuffer(x.nbytes)
+    try:
+        d_in.upload(x)
+        eager = {}
+        for mode in (0, FTI):
+            assert b.apply_device(mode, d_in.ptr, d_out.ptr) == 0
+            assert b.synchronize() == 0
+            eager[mode] = np.empty_like(x)
+            d_out.download(eager[mode])
+        for rep in range(4):
+            for mode in (0, FTI):
+                print('graph replay', rep, 'mode', mode, flush=True)
+                assert b.graph_apply(mode, d_in.ptr, d_out.ptr) == 0
+                assert b.synchronize() == 0
+                y = np.empty_like(x)
+                d_out.download(y)
+                bad = [(i, shape) for i, (shape, a, e) in enumerate(zip(shapes, b.split(y), b.split(eager[mode])))
+                       if not np.array_equal(a, e)]
+                assert not bad, 'arrays differing from the eager result: %s' % bad
+                # eager work between replays
+                assert b.apply_device(1, d_in.ptr, d_out.ptr) == 0
+                assert b.synchronize() == 0
+    finally:
+        b.close()
+
+
+def test_host_apply_matches_device():
+    shapes = [(64, 256), (256, 64), (8, 8), (1024, 32), (128, 128)]
+    arrays = ragged(shapes, 2)
+    x = flat(arrays)
+    b = capi.Batch(shapes)
+    try:
+        for mode in (0, 1, 2, FTI):
+            y = np.empty_like(x)
+            assert b.apply(mode, x, y) == 0
+            assert np.array_equal(y, run_device(b, mode, x))
+    finally:
+        b.close()
+
+
+def test_batch_argument_errors():
+    L = capi.load()
+    hs = np.array([8, 12], np.int32)
+    ws = np.array([8, 8], np.int32)
+    assert not L.bridct_batch_create(2, capi.ptr(hs), capi.ptr(ws))
+    assert not L.bridct_batch_create(0, capi.ptr(hs), capi.ptr(ws))
+    b = capi.Batch([(8, 8), (16, 16)])
+    try:
+        d = capi.DeviceBuffer(4 * 320 + 64)
+        assert b.apply_device(4, d.ptr, d.ptr) == capi.INVALID_ARGUMENT
+        assert b.apply_device(0, d.ptr + 4, d.ptr + 4) == capi.BAD_ALIGNMENT
+        assert b.apply_device(0, d.ptr, d.ptr + 16) == capi.OVERLAPPING_BUFFERS
+        assert b.synchronize() == 0
+    finally:
+        b.close()
