@@ -25,6 +25,10 @@
 namespace bdx {
 namespace {
 
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's vector memory
+// operations, cache write-backs and invalidations included, have completed.
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 constexpr int TH = BDX_BATCH_THREADS;
 constexpr int ROW_OP = BDX_MODE == 0 ? FA : BDX_MODE == 1 ? IA : FA2;
 
@@ -120,13 +124,9 @@ __device__ __forceinline__ void col_any(int h, int w, const float *in, float *ou
     }
 }
 
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's vector memory
-// operations, cache write-backs and invalidations included, have completed.
-__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
-// Waits until *p >= target (thread 0 spins, the block follows), then acquires in every
-// wave.  Producers and consumers may sit on different XCDs, whose L2 caches are not
-// coherent with each other: the agent-scope acquire invalidates the stale lines.
+// Waits until *p >= target: thread 0 spins, then acquires at agent scope (invalidates the
+// stale lines of this CU's L1 and this XCD's L2: producers may sit on other XCDs, whose L2
+// caches are not coherent with this one), and the block follows through the barrier.
 __device__ __forceinline__ void wait_for(unsigned *p, unsigned target, unsigned *err, unsigned limit)
 {
     if (threadIdx.x == 0) {
@@ -138,22 +138,25 @@ __device__ __forceinline__ void wait_for(unsigned *p, unsigned target, unsigned 
             }
             __builtin_amdgcn_s_sleep(2);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        wait_vm();
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    wait_vm();
 }
 
-// Publishes the block's global stores at agent scope, then counts the item as done.  Each
-// wave releases its own stores (waits for them and writes its XCD's L2 back) and waits for
-// the write-back before the barrier; only then does thread 0 count the item.
+// Publishes the block's global stores at agent scope, then counts the item as done: every
+// wave waits until its stores have reached the L2, then thread 0 releases (writes the
+// whole L2 of its XCD back, which covers all waves' stores), waits for the write-back and
+// increments the counter.
 __device__ __forceinline__ void signal_done(unsigned *p)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     wait_vm();
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        wait_vm();
         __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, const bdx_run *__restrict__ runs,
