@@ -6,9 +6,12 @@ arrays at 1024x1024, 512x128, 128x1024 and 32x256, values a smooth field (sum of
 Gaussian blobs, amplitude renormalized into [-1, 1]) plus N(0, 0.01^2) noise.  One step
 is the orthonormal forward DCT-II and then the inverse DCT-III of every batch, with inputs
 already resident in HBM.  Prints one JSON line: whole-job Gpixel/s of forward + inverse,
-per-shape forward / inverse rates, the HBM roofline of the dominant launch (forward over
-the 64 x 1024^2 batch, timed with HIP events on its stream) and the reference CPU path
-timed on this host's cores.
+per-shape forward / inverse rates, the HBM roofline of the dominant kernel (each kernel of
+the 64 x 1024^2 transforms timed alone with HIP events on the plan's stream), a device
+copy as the practical bandwidth reference, and the reference CPU path timed on this host's
+cores.  Secondary legs (not part of `value`): 1,048,576 independent 8x8 blocks in one
+persistent launch vs per-block launches, and a ragged batch of 10,000 arrays of random
+supported shapes (seed 7) in one persistent launch vs one plan launch per array.
 """
 import argparse
 import ctypes
@@ -30,7 +33,7 @@ BATCH = 64
 
 
 class Hip:
-    """The few HIP runtime calls the benchmark needs (events, device sync)."""
+    """The few HIP runtime calls the benchmark needs (events, device sync, device copy)."""
 
     def __init__(self):
         try:
@@ -43,6 +46,8 @@ class Hip:
         self.lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                                  ctypes.c_void_p]
         self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.lib.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                            ctypes.c_void_p]
 
     def event(self):
         e = ctypes.c_void_p()
@@ -60,6 +65,22 @@ class Hip:
 
     def sync(self):
         assert self.lib.hipDeviceSynchronize() == 0
+
+    def copy_d2d(self, dst, src, nbytes, stream=None):
+        assert self.lib.hipMemcpyAsync(dst, src, nbytes, 3, stream) == 0
+
+    def time_ms(self, stream, fn, reps):
+        """Average milliseconds of fn() over reps calls, HIP events on `stream`."""
+        fn()
+        e0, e1 = self.event(), self.event()
+        self.record(e0, stream)
+        for _ in range(reps):
+            fn()
+        self.record(e1, stream)
+        ms = self.elapsed_ms(e0, e1) / reps
+        self.lib.hipEventDestroy(e0)
+        self.lib.hipEventDestroy(e1)
+        return ms
 
 
 def smooth_field(h, w, batch, rng):
@@ -130,18 +151,133 @@ def cpu_baseline(inputs):
                       'thread, median of 5 blocks; host CPU %s' % (per_shape, data.size // 10 ** 6, cpu)}
 
 
+KERNELS_1024 = (
+    # (label, rocprof kernel name, mode, pass)
+    ('row pass, forward', 'k_rows<1024, 1>', capi.FORWARD, 0),
+    ('column pass, forward', 'k_cols<1024, 0>', capi.FORWARD, 1),
+    ('row pass, inverse', 'k_rows<1024, 3>', capi.INVERSE, 0),
+    ('column pass, inverse', 'k_cols<1024, 1>', capi.INVERSE, 1),
+)
+
+
+def committed_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a device copy), or None."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic_1024x1024_batch64.txt')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + ' |'):
+            fields = dict(f.strip().split('=') for f in line.split('|')[1:] if '=' in f)
+            return int(float(fields['traffic_bytes']))
+    return None
+
+
+def leg_copy(hip):
+    n = 256 << 20
+    a, b = capi.DeviceBuffer(n), capi.DeviceBuffer(n)
+    ms = hip.time_ms(None, lambda: hip.copy_d2d(b.ptr, a.ptr, n), 10)
+    return {'bytes': n, 'ms': round(ms, 4), 'gbs': round(2 * n / (ms * 1e-3) / 1e9, 1)}
+
+
+def leg_blocks(hip, lib):
+    """1,048,576 independent 8x8 blocks (an 8192x8192 image tiled, block-contiguous), pixel
+    values integers 0..255 minus 128: one persistent launch (and its hipGraph) for forward,
+    inverse and forward-then-inverse, vs one plan launch per block on a 65,536-block subset."""
+    nb = 1 << 20
+    rng = np.random.default_rng(4)
+    x = (rng.integers(0, 256, nb * 64) - 128).astype(np.float32)
+    batch = capi.Batch([(8, 8)] * nb)
+    da, db = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    da.upload(x)
+    res = {'blocks': nb}
+    for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE), ('fwd_then_inv', capi.FORWARD_THEN_INVERSE)):
+        ms = hip.time_ms(batch.stream, lambda: batch.apply_device(mode, da.ptr, db.ptr), 5)
+        gms = hip.time_ms(batch.stream, lambda: batch.graph_apply(mode, da.ptr, db.ptr), 5)
+        res[name] = {'ms': round(ms, 4), 'gpx_s': round(x.size / (ms * 1e-3) / 1e9, 1),
+                     'graph_ms': round(gms, 4), 'graph_gpx_s': round(x.size / (gms * 1e-3) / 1e9, 1)}
+    assert batch.synchronize() == 0
+    y = np.empty_like(x)
+    db.download(y)
+    res['roundtrip_max_abs_err'] = float(np.abs(y - x).max())
+    plan = capi.Plan(8, 8)
+    sub = 65536
+    hip.sync()
+    t0 = time.perf_counter()
+    for i in range(sub):
+        lib.bridct_plan_apply_device(plan.ptr, capi.FORWARD, da.ptr + 256 * i, db.ptr + 256 * i, 1)
+    assert plan.synchronize() == 0
+    t = time.perf_counter() - t0
+    per_block = sub * 64 / t / 1e9
+    res['per_block_launches_fwd'] = {'subset_blocks': sub, 'gpx_s': round(per_block, 4),
+                                     'extrapolated_s_all_blocks': round(t * nb / sub, 2)}
+    res['persistent_vs_per_block'] = round(res['fwd']['gpx_s'] / per_block, 1)
+    plan.close()
+    batch.close()
+    return res
+
+
+def leg_ragged(hip, lib):
+    """10,000 float32 arrays, each axis uniform over {8, ..., 1024}, values uniform[-1, 1)
+    times a per-array scale 10^U[-3, 3], seed 7: one persistent launch per direction vs one
+    plan launch per array (the same FP32 butterfly kernels)."""
+    rng = np.random.default_rng(7)
+    lengths = [8, 16, 32, 64, 128, 256, 512, 1024]
+    n = 10000
+    shapes = list(zip(rng.choice(lengths, n).tolist(), rng.choice(lengths, n).tolist()))
+    batch = capi.Batch(shapes)
+    x = np.empty(batch.elements, np.float32)
+    for o, (h, w) in zip(batch.offsets, shapes):
+        x[o:o + h * w] = rng.uniform(-1, 1, h * w) * 10 ** rng.uniform(-3, 3)
+    da, db = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    da.upload(x)
+    res = {'arrays': n, 'pixels': int(x.size)}
+    for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
+        ms = hip.time_ms(batch.stream, lambda: batch.apply_device(mode, da.ptr, db.ptr), 3)
+        res[name] = {'ms': round(ms, 3), 'gpx_s': round(x.size / (ms * 1e-3) / 1e9, 1),
+                     'hbm_frac': round(8.0 * x.size / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    gms = hip.time_ms(batch.stream, lambda: batch.graph_apply(capi.FORWARD_THEN_INVERSE, da.ptr, db.ptr), 3)
+    res['graph_fwd_then_inv'] = {'ms': round(gms, 3), 'gpx_s': round(x.size / (gms * 1e-3) / 1e9, 1)}
+    assert batch.synchronize() == 0
+    y = np.empty_like(x)
+    db.download(y)
+    worst = 0.0
+    for o, (h, w) in zip(batch.offsets, shapes):
+        a, b = x[o:o + h * w], y[o:o + h * w]
+        worst = max(worst, float(np.abs(b - a).max() / np.abs(a).max()))
+    res['roundtrip_max_rel_err'] = worst
+    plans = {s: capi.Plan(*s) for s in sorted(set(shapes))}
+    for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
+        hip.sync()
+        t0 = time.perf_counter()
+        for o, s in zip(batch.offsets, shapes):
+            lib.bridct_plan_apply_device(plans[s].ptr, mode, da.ptr + 4 * o, db.ptr + 4 * o, 1)
+        hip.sync()
+        t = time.perf_counter() - t0
+        res['per_array_launches_' + name] = {'s': round(t, 4), 'gpx_s': round(x.size / t / 1e9, 2)}
+    res['batched_vs_plain'] = round((res['fwd']['gpx_s'] + res['inv']['gpx_s']) /
+                                    (res['per_array_launches_fwd']['gpx_s'] + res['per_array_launches_inv']['gpx_s']), 2)
+    for p in plans.values():
+        p.close()
+    batch.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--legs', default='c2,copy,blocks,ragged',
+                    help='comma list of legs: c2 (always run), copy, blocks, ragged')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the transform has no multi-GPU path in this '
               'build (arrays shard across GPUs with no exchange); run with --gpus 1.',
               file=sys.stderr)
         return 2
+    legs = set(args.legs.split(','))
     lib = capi.load()
     if lib.bridct_gpu_available() != 1:
         print('no usable gfx950 GPU', file=sys.stderr)
@@ -175,18 +311,11 @@ def main():
     pixels = sum(h * w * BATCH for h, w in SHAPES)
     value = pixels / (ms_per_step * 1e-3) / 1e9
 
-    # per-shape, per-direction launch times with HIP events on each plan's stream
+    # per-shape, per-direction transform times with HIP events on each plan's stream
     detail = {}
-    reps = 5
     for h, w, plan, (a, b, c) in runs:
-        st = plan.stream
         for name, mode, src, dst in (('fwd', capi.FORWARD, a, b), ('inv', capi.INVERSE, b, c)):
-            e0, e1 = hip.event(), hip.event()
-            hip.record(e0, st)
-            for _ in range(reps):
-                assert plan.apply_device(mode, src.ptr, dst.ptr, BATCH) == 0
-            hip.record(e1, st)
-            ms = hip.elapsed_ms(e0, e1) / reps
+            ms = hip.time_ms(plan.stream, lambda: plan.apply_device(mode, src.ptr, dst.ptr, BATCH), 5)
             gbs = 8.0 * h * w * BATCH / (ms * 1e-3) / 1e9
             detail['%dx%d_%s' % (h, w, name)] = {
                 'ms': round(ms, 4), 'gpx_s': round(h * w * BATCH / (ms * 1e-3) / 1e9, 2),
@@ -197,8 +326,19 @@ def main():
     runs[0][3][2].download(back)
     rt_err = float(np.abs(back - x).max() / np.abs(x).max())
 
-    dom = detail['1024x1024_fwd']
-    achieved = 8.0 * 1024 * 1024 * BATCH / (dom['ms'] * 1e-3) / 1e9
+    # each kernel of the 1024^2 transforms alone: 8 B per pixel (one read, one write)
+    h, w, plan, (a, b, c) = runs[0]
+    px = 1024 * 1024 * BATCH
+    kernels = {}
+    for label, kname, mode, pss in KERNELS_1024:
+        src = a if pss == 0 else b
+        ms = hip.time_ms(plan.stream, lambda: plan.apply_device_pass(mode, pss, src.ptr, b.ptr, BATCH), 5)
+        kernels[kname] = {'what': label, 'ms': round(ms, 4),
+                          'gbs': round(8.0 * px / (ms * 1e-3) / 1e9, 1)}
+    dom_name = max(kernels, key=lambda k: kernels[k]['ms'])
+    dom = kernels[dom_name]
+    achieved = 8.0 * px / (dom['ms'] * 1e-3) / 1e9
+    traffic = committed_traffic(dom_name)
     line = {
         'metric': 'Gpixel/s fwd+inv 2D DCT & %HBM peak; split-MFMA/batched vs plain FP32',
         'value': round(value, 3), 'unit': 'Gpixel/s', 'n_gpus': 1, 'steps': args.steps,
@@ -209,11 +349,18 @@ def main():
                                'then inverse', 'batch': BATCH,
                    'shapes': ['%dx%d' % s for s in SHAPES], 'route': 'butterfly'},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-                     'launch': 'forward, 64 x 1024x1024, 8 B/px algorithmic'},
+                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                     'kernel': dom_name + ' (' + dom['what'] + ', 64 x 1024x1024, 8 B/px per launch)'},
+        'kernels_1024x1024': kernels,
         'per_shape': detail,
         'roundtrip_max_rel_err_1024': rt_err,
     }
+    if 'copy' in legs:
+        line['copy_reference'] = leg_copy(hip)
+    if 'blocks' in legs:
+        line['blocks_8x8'] = leg_blocks(hip, lib)
+    if 'ragged' in legs:
+        line['ragged'] = leg_ragged(hip, lib)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(inputs)
     print(json.dumps(line))

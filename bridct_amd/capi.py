@@ -32,6 +32,8 @@ SIGNATURES = {
                                                 ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_apply_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_plan_apply_device_pass': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                     ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_plan_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_batch_create': (ctypes.c_void_p, [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
@@ -122,6 +124,9 @@ class Plan:
 
     def apply_device(self, mode, d_in, d_out, batch):
         return self.lib.bridct_plan_apply_device(self.ptr, mode, d_in, d_out, batch)
+
+    def apply_device_pass(self, mode, pass_, d_in, d_out, batch):
+        return self.lib.bridct_plan_apply_device_pass(self.ptr, mode, pass_, d_in, d_out, batch)
 
     def apply_batch(self, mode, x, y):
         batch = x.size // (self.h * self.w)

@@ -9,7 +9,12 @@
 namespace bdx {
 namespace {
 
-template <int H> constexpr int col_threads() { return H >= 128 ? 64 * team_size<H>() : kThreads; }
+// One wave per team member, capped at 512 lanes: 1024-lane blocks would cap a lane at 128
+// VGPRs, which the length-1024 column team exceeds (scratch spills).
+template <int H> constexpr int col_threads()
+{
+    return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
+}
 
 template <int W, int OP>
 __global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, const float *__restrict__ tab)

@@ -27,8 +27,8 @@ template <int T> __device__ __forceinline__ float team_mirror(float x)
 }
 
 // LDS copy of the constants of axis length n used by a team; XCH is the per-team LDS
-// region of the forward (FFT exchange, then the output row) and XCHI of the inverse
-// (exchange only), padded so that the teams of one wave start on different banks.
+// region (forward: FFT exchange, then the output row; inverse: the input row, then the
+// FFT exchange), padded so that the teams of one wave start on different banks.
 template <int n> struct RowTab {
     static constexpr int N = n / 2;
     static constexpr int TW = 0;                // team twiddles, 2 (k1 T + t)
@@ -37,7 +37,7 @@ template <int n> struct RowTab {
     static constexpr int AL = 2 * n + N + 4;    // normalization (alpha or alpha^2)
     static constexpr int SIZE = AL + n;
     static constexpr int XCH = n + BDX_TEAM(n);
-    static constexpr int XCHI = N + BDX_TEAM(n);
+    static_assert(SIZE % 4 == 0 && XCH % 4 == 0, "team regions must stay 16-byte aligned");
 };
 
 template <int n, int OP, int THREADS>
@@ -187,6 +187,14 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
 {
     using R = RowTab<n>;
     constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T), C = N1 / 2;
+    // The spectrum is read in pairs (k, n - k) spread over the row: stage the row in the
+    // team's LDS region with whole-line float4 loads first (scattered global loads of half
+    // lines would be fetched twice once the rows in flight outgrow the L2).
+#pragma unroll
+    for (int c = 0; c < n / (4 * T); c++)
+        *reinterpret_cast<float4 *>(xl + 4 * t + 4 * T * c) =
+            *reinterpret_cast<const float4 *>(src + 4 * t + 4 * T * c);
+    team_sync();
     float dr[Q][2][T], di[Q][2][T];
 #pragma unroll
     for (int r = 0; r < Q; r++) {
@@ -200,7 +208,7 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
 #pragma unroll
             for (int w = 0; w < 4; w++) {
                 const int idx = team_out_index<n, T>(g0, g, k2, w);
-                float y = src[idx];
+                float y = xl[idx];
                 if constexpr (OP == IA)
                     y *= lt[R::AL + idx];
                 o[w] = y;
@@ -247,6 +255,7 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
         Dft<T, 1>::run(dr[r][1], di[r][1]);
     }
     __builtin_amdgcn_sched_barrier(0);
+    team_sync();   // every lane has read the staged row before the exchange reuses the region
     float re[N1], im[N1];
 #pragma unroll
     for (int r = 0; r < Q; r++) {

@@ -161,6 +161,22 @@ int bdx_graph_destroy(void *graph_exec)
     return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? 0 : -1;
 }
 
+int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int mode, int pass,
+                       const float *d_tables, void *stream)
+{
+    if (batch == 0)
+        return 0;
+    if ((mode != BDX_FORWARD && mode != BDX_INVERSE) || (pass != 0 && pass != 1))
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (bdx::small_route(h, w))
+        return pass == 0 ? small_dispatch(h, w, d_in, d_out, batch, mode, d_tables, s) : 0;
+    if (pass == 0)
+        return bdx::launch_rows(w, d_in, d_out, batch * (size_t)h, mode == BDX_FORWARD ? bdx::FA : bdx::IA,
+                                d_tables, s);
+    return bdx::launch_cols(h, w, d_in, d_out, batch, mode, d_tables, s);
+}
+
 int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
                   const float *d_tables, void *stream)
 {

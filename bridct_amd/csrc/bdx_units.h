@@ -63,7 +63,7 @@ template <int W, int THREADS> constexpr int row_unit_rows() { return W >= 128 ? 
 template <int W, int OP, int THREADS> constexpr int row_unit_lds()
 {
     if constexpr (W >= 128)
-        return RowTab<W>::SIZE + row_unit_rows<W, THREADS>() * (OP == FA || OP == FA2 ? RowTab<W>::XCH : RowTab<W>::XCHI);
+        return RowTab<W>::SIZE + row_unit_rows<W, THREADS>() * RowTab<W>::XCH;
     else
         return THREADS * (W + 1);
 }
@@ -78,7 +78,7 @@ __device__ __forceinline__ void row_unit(const float *in, float *out, size_t row
     if constexpr (W >= 128) {
         using R = RowTab<W>;
         constexpr int T = team_size<W>();
-        constexpr int X = OP == FA || OP == FA2 ? R::XCH : R::XCHI;
+        constexpr int X = R::XCH;
         float *lt = smem, *xl = smem + R::SIZE;
         if (!tables_ready) {
             load_row_tables<W, OP, THREADS>(lt, tab);

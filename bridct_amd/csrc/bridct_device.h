@@ -104,6 +104,11 @@ int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
  * d_in and d_out must be 16-byte aligned and either identical or disjoint. */
 int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
                   const float *d_tables, void *stream);
+/* One kernel of a FORWARD or INVERSE transform: pass 0 = row pass (d_in -> d_out; the
+ * whole transform for shapes with a whole-array kernel), pass 1 = column pass in place
+ * (d_in == d_out expected; nothing for whole-array shapes). */
+int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int mode, int pass,
+                       const float *d_tables, void *stream);
 
 #ifdef __cplusplus
 }

@@ -242,6 +242,24 @@ int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode, const flo
     return BRIDCT_OK;
 }
 
+int bridct_plan_apply_device_pass(bridct_plan *plan, enum bridct_mode mode, int pass, const float *d_input,
+                                  float *d_output, size_t batch)
+{
+    if (!plan || !d_input || !d_output || (mode != BRIDCT_FORWARD && mode != BRIDCT_INVERSE) ||
+        (pass != 0 && pass != 1) || (pass == 1 && d_input != d_output) ||
+        batch > SIZE_MAX / sizeof(float) / plan->elements)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (((uintptr_t)d_input | (uintptr_t)d_output) % 16)
+        return BRIDCT_BAD_ALIGNMENT;
+    if (d_input != d_output &&
+        distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    if (bdx_transform_pass(d_input, d_output, plan->height, plan->width, batch, (int)mode, pass,
+                           plan->d_tables, plan->stream))
+        return BRIDCT_DEVICE_ERROR;
+    return BRIDCT_OK;
+}
+
 int bridct_plan_synchronize(bridct_plan *plan)
 {
     if (!plan)

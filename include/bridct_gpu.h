@@ -25,6 +25,14 @@ BRIDCT_API int bridct_gpu_available(void);
 BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode,
                                         const float *d_input, float *d_output, size_t batch);
 
+/* One kernel of bridct_plan_apply_device, for profiling a transform kernel by kernel:
+ * pass 0 runs the row pass (d_input -> d_output; for shapes transformed by one whole-array
+ * kernel, the whole transform), pass 1 the column pass in place on d_output (d_input must
+ * equal d_output; nothing to do for whole-array shapes).  FORWARD or INVERSE only.
+ * Pass 0 then pass 1 equals bridct_plan_apply_device. */
+BRIDCT_API int bridct_plan_apply_device_pass(bridct_plan *plan, enum bridct_mode mode, int pass,
+                                             const float *d_input, float *d_output, size_t batch);
+
 /* Mode value accepted by bridct_plan_apply_batch only: FORWARD then INVERSE with the
  * normalized spectrum materialized in FP32 between them (the NumPy Plan.roundtrip
  * semantics of the reference, python/bridct_numpy.c:114-133).  Bitwise equal to the two

@@ -178,3 +178,28 @@ def test_legacy_apply_matches_plan(L, n):
             assert np.array_equal(a, b)
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize('h,w', [(1024, 1024), (512, 128), (8, 1024), (32, 256), (16, 16)])
+def test_passes_compose_to_apply_device(h, w):
+    """Row pass then column pass (bridct_plan_apply_device_pass) equals the whole transform."""
+    batch = 3
+    x = np.random.default_rng(h + w).uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    plan = capi.Plan(h, w)
+    a, b, c = (capi.DeviceBuffer(x.nbytes) for _ in range(3))
+    try:
+        a.upload(x)
+        for mode in (capi.FORWARD, capi.INVERSE):
+            assert plan.apply_device(mode, a.ptr, c.ptr, batch) == 0
+            assert plan.apply_device_pass(mode, 0, a.ptr, b.ptr, batch) == 0
+            assert plan.apply_device_pass(mode, 1, b.ptr, b.ptr, batch) == 0
+            assert plan.synchronize() == 0
+            y, z = np.empty_like(x), np.empty_like(x)
+            b.download(y)
+            c.download(z)
+            assert np.array_equal(y, z)
+        assert plan.apply_device_pass(capi.ROUNDTRIP, 0, a.ptr, b.ptr, batch) == capi.INVALID_ARGUMENT
+        assert plan.apply_device_pass(capi.FORWARD, 2, a.ptr, b.ptr, batch) == capi.INVALID_ARGUMENT
+        assert plan.apply_device_pass(capi.FORWARD, 1, a.ptr, b.ptr, batch) == capi.INVALID_ARGUMENT
+    finally:
+        plan.close()
