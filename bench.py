@@ -9,7 +9,8 @@ already resident in HBM.  Prints one JSON line: whole-job Gpixel/s of forward + 
 per-shape forward / inverse rates, the HBM roofline of the dominant kernel (each kernel of
 the 64 x 1024^2 transforms timed alone with HIP events on the plan's stream), a device
 copy as the practical bandwidth reference, and the reference CPU path timed on this host's
-cores.  Secondary legs (not part of `value`): 1,048,576 independent 8x8 blocks in one
+cores.  Secondary legs (not part of `value`): the dense split-MFMA route vs the
+butterflies at 64^2, 256^2, 1024^2 (batch 128), 1,048,576 independent 8x8 blocks in one
 persistent launch vs per-block launches, and a ragged batch of 10,000 arrays of random
 supported shapes (seed 7) in one persistent launch vs one plan launch per array.
 """
@@ -217,6 +218,61 @@ def leg_blocks(hip, lib):
     return res
 
 
+MFMA_PEAK_TFLOPS = 2500.0   # MI355X dense FP16 / BF16
+
+
+def _dct_fp64(n):
+    """Orthonormal DCT-II matrix in FP64 (docs/API.md:7-13 of the reference), for the
+    error statement of the dense-route leg."""
+    k = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    return np.sqrt(np.where(k == 0, 1.0, 2.0) / n) * np.cos(np.pi * ((k * (2 * j + 1)) % (4 * n)) / (2.0 * n))
+
+
+def leg_dense(hip):
+    """Dense split-MFMA route vs FP32 butterflies at 64^2, 256^2 and 1024^2, batch 128,
+    uniform[-1, 1) inputs: time per direction, MFMA rate (3 products x 2 h w (h + w) flops
+    per array) and max / RMS relative error against FP64 on two arrays per shape."""
+    res = {}
+    names = {capi.ROUTE_BUTTERFLY: 'butterfly_fp32', capi.ROUTE_MFMA_F16: 'mfma_f16_split',
+             capi.ROUTE_MFMA_BF16: 'mfma_bf16_split'}
+    for n in (64, 256, 1024):
+        batch = 128
+        x = np.random.default_rng(n).uniform(-1, 1, (batch, n, n)).astype(np.float32)
+        a, b = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+        a.upload(x)
+        q = _dct_fp64(n)
+        ref = [q @ x[i].astype(np.float64) @ q.T for i in range(2)]
+        shape = {}
+        for route, name in names.items():
+            plan = capi.Plan(n, n)
+            assert plan.set_route(route) == 0
+            rec = {}
+            for d, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
+                ms = hip.time_ms(plan.stream, lambda: plan.apply_device(mode, a.ptr, b.ptr, batch), 3)
+                rec[d + '_ms'] = round(ms, 4)
+                rec[d + '_gpx_s'] = round(x.size / (ms * 1e-3) / 1e9, 1)
+                if route != capi.ROUTE_BUTTERFLY:
+                    tf = 3 * 2.0 * n * n * (2 * n) * batch / (ms * 1e-3) / 1e12
+                    rec[d + '_tflops'] = round(tf, 1)
+                    rec[d + '_mfma_frac'] = round(tf / MFMA_PEAK_TFLOPS, 4)
+            assert plan.apply_device(capi.FORWARD, a.ptr, b.ptr, batch) == 0
+            assert plan.synchronize() == 0
+            y = np.empty_like(x)
+            b.download(y)
+            mx = max(float(np.abs(y[i] - ref[i]).max() / np.abs(ref[i]).max()) for i in range(2))
+            rms = max(float(np.sqrt(np.mean((y[i] - ref[i]) ** 2) / np.mean(ref[i] ** 2))) for i in range(2))
+            rec['fwd_max_rel_err'] = mx
+            rec['fwd_rms_rel_err'] = rms
+            shape[name] = rec
+            plan.close()
+        best = min((shape[nm]['fwd_ms'] + shape[nm]['inv_ms'], nm) for nm in shape)[1]
+        shape['faster_route'] = best
+        res['%dx%d' % (n, n)] = shape
+        del a, b
+    return res
+
+
 def leg_ragged(hip, lib):
     """10,000 float32 arrays, each axis uniform over {8, ..., 1024}, values uniform[-1, 1)
     times a per-array scale 10^U[-3, 3], seed 7: one persistent launch per direction vs one
@@ -269,8 +325,8 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--legs', default='c2,copy,blocks,ragged',
-                    help='comma list of legs: c2 (always run), copy, blocks, ragged')
+    ap.add_argument('--legs', default='c2,copy,dense,blocks,ragged',
+                    help='comma list of legs: c2 (always run), copy, dense, blocks, ragged')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the transform has no multi-GPU path in this '
@@ -357,6 +413,8 @@ def main():
     }
     if 'copy' in legs:
         line['copy_reference'] = leg_copy(hip)
+    if 'dense' in legs:
+        line['dense_route'] = leg_dense(hip)
     if 'blocks' in legs:
         line['blocks_8x8'] = leg_blocks(hip, lib)
     if 'ragged' in legs:

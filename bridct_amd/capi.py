@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbridct_amd.so')
 
 FORWARD, INVERSE, ROUNDTRIP, FORWARD_THEN_INVERSE = 0, 1, 2, 3
+ROUTE_BUTTERFLY, ROUTE_MFMA_F16, ROUTE_MFMA_BF16 = 0, 1, 2
 OK, INVALID_ARGUMENT, WORKSPACE_TOO_SMALL, BAD_ALIGNMENT, OVERLAPPING_BUFFERS, DEVICE_ERROR = (
     0, -1, -2, -3, -4, -5)
 
@@ -35,6 +36,8 @@ SIGNATURES = {
     'bridct_plan_apply_device_pass': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                      ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_plan_set_route': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    'bridct_plan_get_route': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_plan_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_batch_create': (ctypes.c_void_p, [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
     'bridct_batch_destroy': (None, [ctypes.c_void_p]),
@@ -137,6 +140,13 @@ class Plan:
 
     def synchronize(self):
         return self.lib.bridct_plan_synchronize(self.ptr)
+
+    def set_route(self, route):
+        return self.lib.bridct_plan_set_route(self.ptr, route)
+
+    @property
+    def route(self):
+        return self.lib.bridct_plan_get_route(self.ptr)
 
     @property
     def stream(self):

@@ -104,6 +104,17 @@ int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
  * d_in and d_out must be 16-byte aligned and either identical or disjoint. */
 int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
                   const float *d_tables, void *stream);
+/* Dense split-MFMA route: operand precision of the hi/lo split. */
+#define BDX_DENSE_F16 1
+#define BDX_DENSE_BF16 2
+/* Y = scale Q_h' X Q_w'^T (FORWARD) or X = scale Q_h'^T Y Q_w' (INVERSE) for `batch`
+ * contiguous h x w arrays, h, w >= 64, batch <= 65535 and batch h / 64 <= 65535.
+ * d_qh, d_qht, d_qw: the scaled DCT-II matrices Q_h' (h x h), its transpose, Q_w' (w x w),
+ * row-major float32; d_work: batch h w floats for the intermediate; in place allowed. */
+int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, int w, size_t batch, int mode,
+                        int prec, const float *d_qh, const float *d_qht, const float *d_qw, float scale,
+                        void *stream);
+
 /* One kernel of a FORWARD or INVERSE transform: pass 0 = row pass (d_in -> d_out; the
  * whole transform for shapes with a whole-array kernel), pass 1 = column pass in place
  * (d_in == d_out expected; nothing for whole-array shapes). */

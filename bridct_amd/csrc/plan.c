@@ -34,6 +34,11 @@ struct bridct_plan {
     void *stream;         /* hipStream_t */
     float *d_tables;      /* BDX_TABLE_FLOATS */
     float *d_in, *d_out;  /* chunk * elements each */
+    int route;            /* BRIDCT_ROUTE_* */
+    float *d_dense;       /* dense route: Q_h', Q_h'^T, Q_w' */
+    float dense_scale;    /* 1 / (s_h s_w) */
+    float *d_work;        /* dense route intermediate */
+    size_t work_floats;
 };
 
 /* src/plan.c:33-37 */
@@ -114,6 +119,10 @@ void bridct_plan_destroy(bridct_plan *plan)
         return;
     if (plan->stream)
         bdx_stream_sync(plan->stream);
+    if (plan->d_dense)
+        bdx_free(plan->d_dense);
+    if (plan->d_work)
+        bdx_free(plan->d_work);
     if (plan->d_tables)
         bdx_free(plan->d_tables);
     if (plan->d_in)
@@ -174,8 +183,115 @@ static int distance_below(const void *a, const void *b, size_t bytes)
     return (x <= y ? y - x : x - y) < bytes;
 }
 
+/* Power of two near sqrt(n) (2^floor(log2 sqrt n)): the dense route scales Q_n by it so
+ * that the lo parts of the split matrix entries stay normal FP16 numbers. */
+static int dense_matrix_scale(int n)
+{
+    int s = 1;
+    while ((long)(2 * s) * (2 * s) <= n)
+        s *= 2;
+    return s;
+}
+
+/* Q_n'[k][j] = s_n sqrt((k ? 2 : 1) / n) cos(pi k (2j + 1) / (2n)) in FP64, rounded once;
+ * the cosine argument k (2j + 1) is reduced modulo 4n first.  Writes Q' and, when qt is
+ * not NULL, its transpose. */
+static void dense_matrix(int n, float *q, float *qt)
+{
+    const double s = dense_matrix_scale(n);
+    for (int k = 0; k < n; k++) {
+        const double f = s * sqrt((k ? 2.0 : 1.0) / n);
+        for (int j = 0; j < n; j++) {
+            const long m = ((long)k * (2 * j + 1)) % (4L * n);
+            const float v = (float)(f * cos(PI * (double)m / (2.0 * n)));
+            q[(size_t)k * n + j] = v;
+            if (qt)
+                qt[(size_t)j * n + k] = v;
+        }
+    }
+}
+
+int bridct_plan_set_route(bridct_plan *plan, int route)
+{
+    if (!plan || route < BRIDCT_ROUTE_BUTTERFLY || route > BRIDCT_ROUTE_MFMA_BF16)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (route != BRIDCT_ROUTE_BUTTERFLY && (plan->height < 64 || plan->width < 64))
+        return BRIDCT_INVALID_ARGUMENT;
+    if (route != BRIDCT_ROUTE_BUTTERFLY && !plan->d_dense) {
+        const size_t hh = (size_t)plan->height * plan->height, ww = (size_t)plan->width * plan->width;
+        float *host = malloc((2 * hh + ww) * sizeof(float));
+        if (!host)
+            return BRIDCT_DEVICE_ERROR;
+        dense_matrix(plan->height, host, host + hh);
+        dense_matrix(plan->width, host + 2 * hh, NULL);
+        if (bdx_malloc((void **)&plan->d_dense, (2 * hh + ww) * sizeof(float)) ||
+            bdx_copy_h2d(plan->d_dense, host, (2 * hh + ww) * sizeof(float), plan->stream) ||
+            bdx_stream_sync(plan->stream)) {
+            free(host);
+            if (plan->d_dense)
+                bdx_free(plan->d_dense);
+            plan->d_dense = NULL;
+            return BRIDCT_DEVICE_ERROR;
+        }
+        free(host);
+        plan->dense_scale = 1.0f / (float)(dense_matrix_scale(plan->height) * dense_matrix_scale(plan->width));
+    }
+    plan->route = route;
+    return BRIDCT_OK;
+}
+
+int bridct_plan_get_route(const bridct_plan *plan)
+{
+    return plan ? plan->route : BRIDCT_INVALID_ARGUMENT;
+}
+
+/* Dense route: FORWARD / INVERSE through the plan's workspace (grown on demand). */
+static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+{
+    const size_t need = count * plan->elements;
+    if (need > plan->work_floats) {
+        if (plan->d_work && (bdx_stream_sync(plan->stream) || bdx_free(plan->d_work)))
+            return -1;
+        plan->d_work = NULL;
+        plan->work_floats = 0;
+        if (bdx_malloc((void **)&plan->d_work, need * sizeof(float)))
+            return -1;
+        plan->work_floats = need;
+    }
+    const size_t hh = (size_t)plan->height * plan->height;
+    const int prec = plan->route == BRIDCT_ROUTE_MFMA_F16 ? BDX_DENSE_F16 : BDX_DENSE_BF16;
+    /* batches beyond one launch's grid limits go in slices */
+    const size_t max_batch = (size_t)65535 * 64 / (size_t)plan->height < 65535
+                                 ? (size_t)65535 * 64 / (size_t)plan->height : 65535;
+    for (size_t done = 0; done < count;) {
+        const size_t n = count - done < max_batch ? count - done : max_batch;
+        const size_t off = done * plan->elements;
+        if (bdx_dense_transform(d_in + off, d_out + off, plan->d_work, plan->height, plan->width, n, mode, prec,
+                                plan->d_dense, plan->d_dense + hh, plan->d_dense + 2 * hh, plan->dense_scale,
+                                plan->stream))
+            return -1;
+        done += n;
+    }
+    return 0;
+}
+
+static int transform_mode(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+{
+    if (plan->route == BRIDCT_ROUTE_BUTTERFLY)
+        return bdx_transform(d_in, d_out, plan->height, plan->width, count, mode, plan->d_tables, plan->stream);
+    if (mode == BDX_ROUNDTRIP)
+        return dense(plan, BDX_FORWARD, d_in, d_out, count) || dense(plan, BDX_INVERSE, d_out, d_out, count);
+    return dense(plan, mode, d_in, d_out, count);
+}
+
 static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
 {
+    if (plan->route != BRIDCT_ROUTE_BUTTERFLY) {
+        if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
+            return transform_mode(plan, BDX_FORWARD, d_in, d_out, count) ||
+                   transform_mode(plan, BDX_INVERSE, d_out, d_out, count);
+        return transform_mode(plan, mode, d_in, d_out, count);
+    }
     if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
         return bdx_transform(d_in, d_out, plan->height, plan->width, count, BDX_FORWARD,
                              plan->d_tables, plan->stream) ||
@@ -236,8 +352,7 @@ int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode, const flo
     if (d_input != d_output &&
         distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
         return BRIDCT_OVERLAPPING_BUFFERS;
-    if (bdx_transform(d_input, d_output, plan->height, plan->width, batch, (int)mode,
-                      plan->d_tables, plan->stream))
+    if (transform(plan, (int)mode, d_input, d_output, batch))
         return BRIDCT_DEVICE_ERROR;
     return BRIDCT_OK;
 }

@@ -25,6 +25,20 @@ BRIDCT_API int bridct_gpu_available(void);
 BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode,
                                         const float *d_input, float *d_output, size_t batch);
 
+/* Routes of a plan.  BUTTERFLY (default): the FP32 Shao-Johnson butterflies.
+ * MFMA_F16 / MFMA_BF16: the dense route Y = C_h X C_w^T on the matrix cores, both
+ * operands split into hi + lo half-precision parts (FP16 or BF16), three products, FP32
+ * accumulation; both axes must be at least 64.  Its error against the FP64 definition is
+ * larger than the butterflies' (measured per shape); the FP16 split needs inputs of
+ * moderate range (|x| up to about 30 and not uniformly tiny, or half-precision overflow
+ * and underflow of the split parts degrade it).  The dense route keeps an intermediate
+ * buffer of batch h w floats in the plan, allocated on the first call that needs it. */
+#define BRIDCT_ROUTE_BUTTERFLY 0
+#define BRIDCT_ROUTE_MFMA_F16 1
+#define BRIDCT_ROUTE_MFMA_BF16 2
+BRIDCT_API int bridct_plan_set_route(bridct_plan *plan, int route);
+BRIDCT_API int bridct_plan_get_route(const bridct_plan *plan);
+
 /* One kernel of bridct_plan_apply_device, for profiling a transform kernel by kernel:
  * pass 0 runs the row pass (d_input -> d_output; for shapes transformed by one whole-array
  * kernel, the whole transform), pass 1 the column pass in place on d_output (d_input must
