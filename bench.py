@@ -139,6 +139,18 @@ def cpu_baseline(inputs):
     if rc:
         return None
     gpx = data.size / ns.value
+    # the plain 256x256 method: 256 arrays (seeds 0..255), forward then inverse
+    c1 = c1_inputs()
+    n1 = c1.shape[0]
+    c1_h = np.full(n1, 256, np.int32)
+    c1_o = (np.arange(n1, dtype=np.uint64) * 65536)
+    c1_out, c1_spec = np.empty_like(c1), np.empty_like(c1)
+    c1_rates = {}
+    for t in (1, threads):
+        ns1 = ctypes.c_double()
+        if lib.ref_bench_run(n1, c1_h.ctypes.data, c1_h.ctypes.data, c1_o.ctypes.data, c1.ctypes.data,
+                             c1_out.ctypes.data, c1_spec.ctypes.data, 3, t, 5, 200.0, ctypes.byref(ns1)) == 0:
+            c1_rates['threads_%d' % t] = round(c1.size / ns1.value, 4)
     cpu = 'unknown'
     try:
         for line in open('/proc/cpuinfo'):
@@ -149,7 +161,8 @@ def cpu_baseline(inputs):
         pass
     return {'value': round(gpx, 4), 'unit': 'Gpixel/s', 'cores': threads, 'kind': 'reference',
             'sample': '%d arrays of each shape (%d Mpx), forward then inverse, one plan per '
-                      'thread, median of 5 blocks; host CPU %s' % (per_shape, data.size // 10 ** 6, cpu)}
+                      'thread, median of 5 blocks; host CPU %s' % (per_shape, data.size // 10 ** 6, cpu),
+            'plain_256x256_gpx_s': c1_rates}
 
 
 KERNELS_1024 = (
@@ -215,6 +228,66 @@ def leg_blocks(hip, lib):
     res['persistent_vs_per_block'] = round(res['fwd']['gpx_s'] / per_block, 1)
     plan.close()
     batch.close()
+    return res
+
+
+def c1_inputs():
+    """256 float32 arrays of 256x256, i.i.d. uniform[-1, 1), array i from seed i."""
+    return np.stack([np.random.default_rng(i).uniform(-1, 1, (256, 256)).astype(np.float32)
+                     for i in range(256)])
+
+
+def leg_c1(hip, lib):
+    """Plain method: 256x256 float32 arrays, uniform[-1, 1) from seeds 0..255, forward DCT-II
+    then inverse DCT-III (spectrum materialized), FP32 butterflies: one batched launch per
+    direction and one launch per array; max relative errors against FP64."""
+    x = c1_inputs()
+    n = x.shape[0]
+    a, b, c = (capi.DeviceBuffer(x.nbytes) for _ in range(3))
+    a.upload(x)
+    plan = capi.Plan(256, 256)
+    res = {'arrays': n}
+
+    def both():
+        assert plan.apply_device(capi.FORWARD, a.ptr, b.ptr, n) == 0
+        assert plan.apply_device(capi.INVERSE, b.ptr, c.ptr, n) == 0
+
+    ms = hip.time_ms(plan.stream, both, 5)
+    res['batched'] = {'ms_fwd_plus_inv': round(ms, 4), 'gpx_s': round(x.size / (ms * 1e-3) / 1e9, 1)}
+    pp = plan.ptr
+    step = 256 * 256 * 4
+
+    def per_array():
+        for i in range(n):
+            lib.bridct_plan_apply_device(pp, capi.FORWARD, a.ptr + step * i, b.ptr + step * i, 1)
+            lib.bridct_plan_apply_device(pp, capi.INVERSE, b.ptr + step * i, c.ptr + step * i, 1)
+
+    hip.sync()
+    t0 = time.perf_counter()
+    per_array()
+    assert plan.synchronize() == 0
+    t = time.perf_counter() - t0
+    res['per_array_launches'] = {'s': round(t, 4), 'gpx_s': round(x.size / t / 1e9, 2)}
+    y, z = np.empty_like(x), np.empty_like(x)
+    b.download(y)
+    c.download(z)
+    q = _dct_fp64(256)
+    ef = ei = er = 0.0
+    for i in range(n):
+        x64 = x[i].astype(np.float64)
+        yr = q @ x64 @ q.T
+        ir = q.T @ x64 @ q
+        ef = max(ef, float(np.abs(y[i] - yr).max() / np.abs(yr).max()))
+        er = max(er, float(np.abs(z[i] - x64).max() / np.abs(x64).max()))
+        if i < 32:
+            assert plan.apply_device(capi.INVERSE, a.ptr + step * i, b.ptr + step * i, 1) == 0
+    assert plan.synchronize() == 0
+    b.download(y)
+    for i in range(32):
+        ir = q.T @ x[i].astype(np.float64) @ q
+        ei = max(ei, float(np.abs(y[i] - ir).max() / np.abs(ir).max()))
+    res['max_rel_err_vs_fp64'] = {'forward': ef, 'inverse_32_arrays': ei, 'forward_then_inverse': er}
+    plan.close()
     return res
 
 
@@ -325,8 +398,8 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--legs', default='c2,copy,dense,blocks,ragged',
-                    help='comma list of legs: c2 (always run), copy, dense, blocks, ragged')
+    ap.add_argument('--legs', default='c2,copy,c1,dense,blocks,ragged',
+                    help='comma list of legs: c2 (always run), copy, c1, dense, blocks, ragged')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the transform has no multi-GPU path in this '
@@ -413,6 +486,8 @@ def main():
     }
     if 'copy' in legs:
         line['copy_reference'] = leg_copy(hip)
+    if 'c1' in legs:
+        line['plain_256x256'] = leg_c1(hip, lib)
     if 'dense' in legs:
         line['dense_route'] = leg_dense(hip)
     if 'blocks' in legs:

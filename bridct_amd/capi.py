@@ -38,6 +38,8 @@ SIGNATURES = {
     'bridct_plan_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_plan_set_route': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     'bridct_plan_get_route': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_plan_set_pack': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    'bridct_plan_get_pack': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_plan_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_batch_create': (ctypes.c_void_p, [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
     'bridct_batch_destroy': (None, [ctypes.c_void_p]),
@@ -147,6 +149,13 @@ class Plan:
     @property
     def route(self):
         return self.lib.bridct_plan_get_route(self.ptr)
+
+    def set_pack(self, pack):
+        return self.lib.bridct_plan_set_pack(self.ptr, pack)
+
+    @property
+    def pack(self):
+        return self.lib.bridct_plan_get_pack(self.ptr)
 
     @property
     def stream(self):

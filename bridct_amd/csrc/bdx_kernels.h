@@ -24,14 +24,22 @@ constexpr int rows_per_block(int w) { return kPassLds / w; }
 constexpr int cols_per_block(int h) { return kPassLds / h > 256 ? 256 : kPassLds / h; }
 
 // mode: 0 forward, 1 inverse, 2 fused round trip.  Return 0 or a negative value.
-int launch_small_h8(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h16(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h32(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h64(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h128(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h256(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h512(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
-int launch_small_h1024(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s);
+int launch_small_h8(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h16(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h32(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h64(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h128(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h256(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h512(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
+int launch_small_h1024(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
+                      hipStream_t s);
 
 // Row pass over nrows rows of length w: op 1 = forward*alpha, 2 = forward*alpha^2,
 // 3 = alpha*inverse, 4 = inverse.

@@ -14,19 +14,19 @@
 
 namespace {
 
-int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
-                   hipStream_t s)
+int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int pack, int mode,
+                   const float *tab, hipStream_t s)
 {
     using namespace bdx;
     switch (h) {
-    case 8: return launch_small_h8(w, in, out, batch, mode, tab, s);
-    case 16: return launch_small_h16(w, in, out, batch, mode, tab, s);
-    case 32: return launch_small_h32(w, in, out, batch, mode, tab, s);
-    case 64: return launch_small_h64(w, in, out, batch, mode, tab, s);
-    case 128: return launch_small_h128(w, in, out, batch, mode, tab, s);
-    case 256: return launch_small_h256(w, in, out, batch, mode, tab, s);
-    case 512: return launch_small_h512(w, in, out, batch, mode, tab, s);
-    case 1024: return launch_small_h1024(w, in, out, batch, mode, tab, s);
+    case 8: return launch_small_h8(w, in, out, batch, pack, mode, tab, s);
+    case 16: return launch_small_h16(w, in, out, batch, pack, mode, tab, s);
+    case 32: return launch_small_h32(w, in, out, batch, pack, mode, tab, s);
+    case 64: return launch_small_h64(w, in, out, batch, pack, mode, tab, s);
+    case 128: return launch_small_h128(w, in, out, batch, pack, mode, tab, s);
+    case 256: return launch_small_h256(w, in, out, batch, pack, mode, tab, s);
+    case 512: return launch_small_h512(w, in, out, batch, pack, mode, tab, s);
+    case 1024: return launch_small_h1024(w, in, out, batch, pack, mode, tab, s);
     default: return -1;
     }
 }
@@ -161,8 +161,8 @@ int bdx_graph_destroy(void *graph_exec)
     return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? 0 : -1;
 }
 
-int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int mode, int pass,
-                       const float *d_tables, void *stream)
+int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
+                       int pass, const float *d_tables, void *stream)
 {
     if (batch == 0)
         return 0;
@@ -170,14 +170,14 @@ int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t bat
         return -1;
     hipStream_t s = (hipStream_t)stream;
     if (bdx::small_route(h, w))
-        return pass == 0 ? small_dispatch(h, w, d_in, d_out, batch, mode, d_tables, s) : 0;
+        return pass == 0 ? small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s) : 0;
     if (pass == 0)
         return bdx::launch_rows(w, d_in, d_out, batch * (size_t)h, mode == BDX_FORWARD ? bdx::FA : bdx::IA,
                                 d_tables, s);
     return bdx::launch_cols(h, w, d_in, d_out, batch, mode, d_tables, s);
 }
 
-int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
+int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
                   const float *d_tables, void *stream)
 {
     if (batch == 0)
@@ -186,7 +186,7 @@ int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, i
         return -1;
     hipStream_t s = (hipStream_t)stream;
     if (bdx::small_route(h, w))
-        return small_dispatch(h, w, d_in, d_out, batch, mode, d_tables, s);
+        return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s);
     const size_t nrows = batch * (size_t)h;
     const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;
     if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s) ||

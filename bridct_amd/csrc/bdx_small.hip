@@ -23,15 +23,16 @@ __global__ __launch_bounds__(kThreads) void k_small(const float *in, float *out,
 }
 
 template <int H, int W>
-int launch_hw(const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s)
+int launch_hw(const float *in, float *out, size_t batch, int pack, int mode, const float *tab, hipStream_t s)
 {
     if constexpr (!small_route(H, W)) {
         return -1;
     } else {
         constexpr int P = W + 1;
         constexpr int per_max = kSmallLds / (H * P);
-        // enough workgroups to cover the GPU several times before packing arrays together
-        size_t per = (batch + 1023) / 1024;
+        // automatic (pack 0): enough workgroups to cover the GPU several times before
+        // packing arrays together; otherwise the requested arrays per workgroup
+        size_t per = pack > 0 ? (size_t)pack : (batch + 1023) / 1024;
         if (per < 1)
             per = 1;
         if (per > (size_t)per_max)
@@ -61,18 +62,18 @@ int launch_hw(const float *in, float *out, size_t batch, int mode, const float *
 #define BDX_CAT2(a, b) a##b
 #define BDX_CAT(a, b) BDX_CAT2(a, b)
 
-int BDX_CAT(launch_small_h, BDX_H)(int w, const float *in, float *out, size_t batch, int mode,
+int BDX_CAT(launch_small_h, BDX_H)(int w, const float *in, float *out, size_t batch, int pack, int mode,
                                     const float *tab, hipStream_t s)
 {
     switch (w) {
-    case 8: return launch_hw<BDX_H, 8>(in, out, batch, mode, tab, s);
-    case 16: return launch_hw<BDX_H, 16>(in, out, batch, mode, tab, s);
-    case 32: return launch_hw<BDX_H, 32>(in, out, batch, mode, tab, s);
-    case 64: return launch_hw<BDX_H, 64>(in, out, batch, mode, tab, s);
-    case 128: return launch_hw<BDX_H, 128>(in, out, batch, mode, tab, s);
-    case 256: return launch_hw<BDX_H, 256>(in, out, batch, mode, tab, s);
-    case 512: return launch_hw<BDX_H, 512>(in, out, batch, mode, tab, s);
-    case 1024: return launch_hw<BDX_H, 1024>(in, out, batch, mode, tab, s);
+    case 8: return launch_hw<BDX_H, 8>(in, out, batch, pack, mode, tab, s);
+    case 16: return launch_hw<BDX_H, 16>(in, out, batch, pack, mode, tab, s);
+    case 32: return launch_hw<BDX_H, 32>(in, out, batch, pack, mode, tab, s);
+    case 64: return launch_hw<BDX_H, 64>(in, out, batch, pack, mode, tab, s);
+    case 128: return launch_hw<BDX_H, 128>(in, out, batch, pack, mode, tab, s);
+    case 256: return launch_hw<BDX_H, 256>(in, out, batch, pack, mode, tab, s);
+    case 512: return launch_hw<BDX_H, 512>(in, out, batch, pack, mode, tab, s);
+    case 1024: return launch_hw<BDX_H, 1024>(in, out, batch, pack, mode, tab, s);
     default: return -1;
     }
 }

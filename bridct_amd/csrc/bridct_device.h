@@ -101,8 +101,9 @@ int bdx_copy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Orthonormal 2D transform of `batch` contiguous row-major h x w float32 arrays.
- * d_in and d_out must be 16-byte aligned and either identical or disjoint. */
-int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int mode,
+ * d_in and d_out must be 16-byte aligned and either identical or disjoint.  pack: arrays
+ * per workgroup of the whole-array kernel (0 = automatic; ignored by two-pass shapes). */
+int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
                   const float *d_tables, void *stream);
 /* Dense split-MFMA route: operand precision of the hi/lo split. */
 #define BDX_DENSE_F16 1
@@ -118,8 +119,8 @@ int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, i
 /* One kernel of a FORWARD or INVERSE transform: pass 0 = row pass (d_in -> d_out; the
  * whole transform for shapes with a whole-array kernel), pass 1 = column pass in place
  * (d_in == d_out expected; nothing for whole-array shapes). */
-int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int mode, int pass,
-                       const float *d_tables, void *stream);
+int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
+                       int pass, const float *d_tables, void *stream);
 
 #ifdef __cplusplus
 }

@@ -7,4 +7,10 @@
  * evaluated in FP64 and rounded once. */
 __attribute__((visibility("hidden"))) void bridct_build_tables(float *t);
 
+/* Route and whole-array packing of shape (h, w) from the schedule table; 0 if found. */
+__attribute__((visibility("hidden"))) int bridct_schedule_lookup(int h, int w, int *route, int *pack);
+
+/* The same lookup in the table file `path` (exported for tests of the table format). */
+int bdx_schedule_lookup_file(const char *path, int h, int w, int *route, int *pack);
+
 #endif

@@ -35,6 +35,7 @@ struct bridct_plan {
     float *d_tables;      /* BDX_TABLE_FLOATS */
     float *d_in, *d_out;  /* chunk * elements each */
     int route;            /* BRIDCT_ROUTE_* */
+    int pack;             /* arrays per workgroup of the whole-array kernel, 0 = automatic */
     float *d_dense;       /* dense route: Q_h', Q_h'^T, Q_w' */
     float dense_scale;    /* 1 / (s_h s_w) */
     float *d_work;        /* dense route intermediate */
@@ -161,6 +162,14 @@ bridct_plan *bridct_plan_create(int height, int width)
         goto fail;
     }
     free(host);
+    /* per-shape route and packing chosen by the autotuner, when a table is present; a
+     * route this plan cannot take (e.g. a table written for other shapes) is ignored */
+    int route = BRIDCT_ROUTE_BUTTERFLY, pack = 0;
+    if (bridct_schedule_lookup(height, width, &route, &pack) == 0) {
+        plan->pack = pack;
+        if (route != BRIDCT_ROUTE_BUTTERFLY && bridct_plan_set_route(plan, route) != BRIDCT_OK)
+            plan->route = BRIDCT_ROUTE_BUTTERFLY;
+    }
     return plan;
 
 fail:
@@ -245,6 +254,19 @@ int bridct_plan_get_route(const bridct_plan *plan)
     return plan ? plan->route : BRIDCT_INVALID_ARGUMENT;
 }
 
+int bridct_plan_set_pack(bridct_plan *plan, int pack)
+{
+    if (!plan || pack < 0 || pack > 4096)
+        return BRIDCT_INVALID_ARGUMENT;
+    plan->pack = pack;
+    return BRIDCT_OK;
+}
+
+int bridct_plan_get_pack(const bridct_plan *plan)
+{
+    return plan ? plan->pack : BRIDCT_INVALID_ARGUMENT;
+}
+
 /* Dense route: FORWARD / INVERSE through the plan's workspace (grown on demand). */
 static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
 {
@@ -278,7 +300,8 @@ static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, s
 static int transform_mode(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
 {
     if (plan->route == BRIDCT_ROUTE_BUTTERFLY)
-        return bdx_transform(d_in, d_out, plan->height, plan->width, count, mode, plan->d_tables, plan->stream);
+        return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, mode, plan->d_tables,
+                             plan->stream);
     if (mode == BDX_ROUNDTRIP)
         return dense(plan, BDX_FORWARD, d_in, d_out, count) || dense(plan, BDX_INVERSE, d_out, d_out, count);
     return dense(plan, mode, d_in, d_out, count);
@@ -293,11 +316,11 @@ static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_ou
         return transform_mode(plan, mode, d_in, d_out, count);
     }
     if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
-        return bdx_transform(d_in, d_out, plan->height, plan->width, count, BDX_FORWARD,
+        return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, BDX_FORWARD,
                              plan->d_tables, plan->stream) ||
-               bdx_transform(d_out, d_out, plan->height, plan->width, count, BDX_INVERSE,
+               bdx_transform(d_out, d_out, plan->height, plan->width, count, plan->pack, BDX_INVERSE,
                              plan->d_tables, plan->stream);
-    return bdx_transform(d_in, d_out, plan->height, plan->width, count, mode, plan->d_tables,
+    return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, mode, plan->d_tables,
                          plan->stream);
 }
 
@@ -369,7 +392,7 @@ int bridct_plan_apply_device_pass(bridct_plan *plan, enum bridct_mode mode, int 
     if (d_input != d_output &&
         distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
         return BRIDCT_OVERLAPPING_BUFFERS;
-    if (bdx_transform_pass(d_input, d_output, plan->height, plan->width, batch, (int)mode, pass,
+    if (bdx_transform_pass(d_input, d_output, plan->height, plan->width, batch, plan->pack, (int)mode, pass,
                            plan->d_tables, plan->stream))
         return BRIDCT_DEVICE_ERROR;
     return BRIDCT_OK;

@@ -39,6 +39,16 @@ BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode
 BRIDCT_API int bridct_plan_set_route(bridct_plan *plan, int route);
 BRIDCT_API int bridct_plan_get_route(const bridct_plan *plan);
 
+/* Arrays per workgroup of the whole-array kernel (shapes whose array fits one
+ * workgroup's LDS): 0 = automatic, else 1..4096 (clamped to what fits). */
+BRIDCT_API int bridct_plan_set_pack(bridct_plan *plan, int pack);
+BRIDCT_API int bridct_plan_get_pack(const bridct_plan *plan);
+
+/* A new plan takes its route and packing from the schedule table written by the
+ * autotuner (bridct_amd/autotune.py): the file named by $BRIDCT_SCHEDULE (empty or
+ * "none" for no table), else schedule_gfx950.txt beside the library.  Lines read
+ * "<height> <width> <butterfly|mfma_f16|mfma_bf16> <pack>". */
+
 /* One kernel of bridct_plan_apply_device, for profiling a transform kernel by kernel:
  * pass 0 runs the row pass (d_input -> d_output; for shapes transformed by one whole-array
  * kernel, the whole transform), pass 1 the column pass in place on d_output (d_input must
