@@ -14,6 +14,12 @@
  *
  * The whole batch goes to the GPU in one call (bridct_plan_apply_batch), not one call
  * per image as in the reference loop (python/bridct_numpy.c:114-133).
+ *
+ * RaggedPlan(shapes) has no counterpart in the reference: a list of arrays of mixed
+ * supported shapes goes through one persistent GPU launch per direction (bridct_batch_*).
+ * forward / inverse / roundtrip(arrays, *, out=None) take a list or tuple of float32
+ * (h_i, w_i) arrays, validated like Plan inputs, and return a list (or fill `out`, a list
+ * of writable arrays of the same shapes, each disjoint from its input).
  */
 #define PY_SSIZE_T_CLEAN
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
@@ -24,6 +30,8 @@
 #include "bridct_gpu.h"
 
 #include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
 
 #ifdef Py_GIL_DISABLED
 #error "bridct_numpy needs a CPython build with the GIL."
@@ -260,6 +268,335 @@ static PyTypeObject PlanType = {
     .tp_init = (initproc)Plan_init,
     .tp_new = PyType_GenericNew};
 
+/* ---- RaggedPlan ------------------------------------------------------------------ */
+
+typedef struct {
+    PyObject_HEAD
+    bridct_batch *batch;
+    PyObject *shapes;           /* tuple of (h, w) tuples */
+    size_t count, elements;
+    float *stage_in, *stage_out;   /* host staging, allocated on first use */
+    int initialized, busy;
+} RaggedPlan;
+
+static void ragged_release(RaggedPlan *self)
+{
+    bridct_batch_destroy(self->batch);
+    self->batch = NULL;
+    free(self->stage_in);
+    free(self->stage_out);
+    self->stage_in = self->stage_out = NULL;
+}
+
+static int RaggedPlan_init(RaggedPlan *self, PyObject *args, PyObject *kwargs)
+{
+    static char *keywords[] = {"shapes", NULL};
+    PyObject *seq = NULL, *shapes = NULL;
+    int *hs = NULL, *ws = NULL;
+    if (self->initialized) {
+        PyErr_SetString(PyExc_RuntimeError, "A RaggedPlan cannot be reinitialized.");
+        return -1;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is already initializing or executing.");
+        return -1;
+    }
+    self->busy = 1;
+    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "O:RaggedPlan", keywords, &seq))
+        goto fail;
+    seq = PySequence_Fast(seq, "shapes must be a sequence of (height, width) pairs.");
+    if (!seq)
+        goto fail;
+    const Py_ssize_t n = PySequence_Fast_GET_SIZE(seq);
+    if (n < 1) {
+        PyErr_SetString(PyExc_ValueError, "A RaggedPlan needs at least one shape.");
+        goto fail;
+    }
+    hs = malloc((size_t)n * sizeof(int));
+    ws = malloc((size_t)n * sizeof(int));
+    shapes = PyTuple_New(n);
+    if (!hs || !ws || !shapes) {
+        PyErr_NoMemory();
+        goto fail;
+    }
+    for (Py_ssize_t i = 0; i < n; i++) {
+        int h, w;
+        PyObject *item = PySequence_Fast_GET_ITEM(seq, i);
+        if (!PyTuple_Check(item) && !PyList_Check(item)) {
+            PyErr_SetString(PyExc_TypeError, "Each shape must be a (height, width) pair.");
+            goto fail;
+        }
+        PyObject *pair = PySequence_Tuple(item);
+        if (!pair)
+            goto fail;
+        const int ok = PyArg_ParseTuple(pair, "ii", &h, &w);
+        Py_DECREF(pair);
+        if (!ok)
+            goto fail;
+        if (!bridct_supported(h, w)) {
+            PyErr_SetString(PyExc_ValueError, "Both dimensions must be powers of two from 8 through 1024.");
+            goto fail;
+        }
+        hs[i] = h;
+        ws[i] = w;
+        PyObject *t = Py_BuildValue("(ii)", h, w);
+        if (!t)
+            goto fail;
+        PyTuple_SET_ITEM(shapes, i, t);
+    }
+    if (!bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT GPU batch cannot be created.");
+        goto fail;
+    }
+    self->batch = bridct_batch_create((size_t)n, hs, ws);
+    if (!self->batch) {
+        PyErr_NoMemory();
+        goto fail;
+    }
+    self->count = (size_t)n;
+    self->elements = bridct_batch_elements(self->batch);
+    self->shapes = shapes;
+    self->initialized = 1;
+    self->busy = 0;
+    free(hs);
+    free(ws);
+    Py_DECREF(seq);
+    return 0;
+fail:
+    self->busy = 0;
+    free(hs);
+    free(ws);
+    Py_XDECREF(shapes);
+    Py_XDECREF(seq);
+    return -1;
+}
+
+static void RaggedPlan_dealloc(RaggedPlan *self)
+{
+    ragged_release(self);
+    Py_XDECREF(self->shapes);
+    Py_TYPE(self)->tp_free((PyObject *)self);
+}
+
+static PyObject *RaggedPlan_close(RaggedPlan *self, PyObject *unused)
+{
+    (void)unused;
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError,
+                        "This RaggedPlan is already initializing or executing, possibly in another thread.");
+        return NULL;
+    }
+    ragged_release(self);
+    Py_RETURN_NONE;
+}
+
+/* Validates array i of a list against shape i (the Plan rules, for a 2D array). */
+static PyArrayObject *ragged_item(PyObject *object, int h, int w, int writable)
+{
+    if (!PyArray_CheckExact(object)) {
+        PyErr_SetString(PyExc_TypeError, "Expected exact NumPy ndarrays; no implicit conversion is performed.");
+        return NULL;
+    }
+    PyArrayObject *a = (PyArrayObject *)object;
+    if (PyArray_TYPE(a) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(a)) {
+        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32.");
+        return NULL;
+    }
+    if (PyArray_NDIM(a) != 2 || PyArray_DIM(a, 0) != h || PyArray_DIM(a, 1) != w) {
+        PyErr_SetString(PyExc_ValueError, "Each array must have the planned (height, width).");
+        return NULL;
+    }
+    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
+        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, float-aligned storage.");
+        return NULL;
+    }
+    if (writable && !PyArray_ISWRITEABLE(a)) {
+        PyErr_SetString(PyExc_ValueError, "The output arrays must be writable.");
+        return NULL;
+    }
+    return a;
+}
+
+static PyObject *ragged_execute(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
+                                int mode)
+{
+    if (!self->batch) {
+        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is closed or uninitialized.");
+        return NULL;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is executing in another thread; use one per thread.");
+        return NULL;
+    }
+    self->busy = 1;
+    PyObject *in_seq = NULL, *out_seq = NULL, *result = NULL;
+    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
+    if (nargs != 1 || nkw > 1) {
+        PyErr_SetString(PyExc_TypeError, "Use method(arrays, *, out=None).");
+        goto fail;
+    }
+    PyObject *out_obj = Py_None;
+    if (nkw) {
+        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
+            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
+            goto fail;
+        }
+        out_obj = args[1];
+    }
+    if (!PyList_Check(args[0]) && !PyTuple_Check(args[0])) {
+        PyErr_SetString(PyExc_TypeError, "Expected a list or tuple of arrays.");
+        goto fail;
+    }
+    in_seq = PySequence_Fast(args[0], "Expected a list or tuple of arrays.");
+    if (!in_seq)
+        goto fail;
+    if ((size_t)PySequence_Fast_GET_SIZE(in_seq) != self->count) {
+        PyErr_SetString(PyExc_ValueError, "The list length must equal the number of planned shapes.");
+        goto fail;
+    }
+    if (out_obj != Py_None) {
+        if (!PyList_Check(out_obj) && !PyTuple_Check(out_obj)) {
+            PyErr_SetString(PyExc_TypeError, "out must be a list or tuple of arrays.");
+            goto fail;
+        }
+        out_seq = PySequence_Fast(out_obj, "out must be a list or tuple of arrays.");
+        if (!out_seq)
+            goto fail;
+        if ((size_t)PySequence_Fast_GET_SIZE(out_seq) != self->count) {
+            PyErr_SetString(PyExc_ValueError, "out must hold one array per planned shape.");
+            goto fail;
+        }
+    }
+    for (size_t i = 0; i < self->count; i++) {
+        PyObject *shape = PyTuple_GET_ITEM(self->shapes, (Py_ssize_t)i);
+        const int h = (int)PyLong_AsLong(PyTuple_GET_ITEM(shape, 0)), w = (int)PyLong_AsLong(PyTuple_GET_ITEM(shape, 1));
+        PyArrayObject *a = ragged_item(PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i), h, w, 0);
+        if (!a)
+            goto fail;
+        if (out_seq) {
+            PyArrayObject *o = ragged_item(PySequence_Fast_GET_ITEM(out_seq, (Py_ssize_t)i), h, w, 1);
+            if (!o)
+                goto fail;
+            const uintptr_t x = (uintptr_t)PyArray_DATA(a), y = (uintptr_t)PyArray_DATA(o);
+            if ((x <= y ? y - x : x - y) < (size_t)PyArray_NBYTES(a)) {
+                PyErr_SetString(PyExc_ValueError, "Input and output arrays must not overlap.");
+                goto fail;
+            }
+        }
+    }
+    if (!self->stage_in) {
+        self->stage_in = malloc(self->elements * sizeof(float));
+        self->stage_out = malloc(self->elements * sizeof(float));
+        if (!self->stage_in || !self->stage_out) {
+            free(self->stage_in);
+            free(self->stage_out);
+            self->stage_in = self->stage_out = NULL;
+            PyErr_NoMemory();
+            goto fail;
+        }
+    }
+    for (size_t i = 0; i < self->count; i++) {
+        PyArrayObject *a = (PyArrayObject *)PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i);
+        memcpy(self->stage_in + bridct_batch_offset(self->batch, i), PyArray_DATA(a), (size_t)PyArray_NBYTES(a));
+    }
+    int status;
+    if (self->elements >= 16384) {
+        Py_BEGIN_ALLOW_THREADS
+        status = bridct_batch_apply(self->batch, mode, self->stage_in, self->stage_out);
+        Py_END_ALLOW_THREADS
+    } else {
+        status = bridct_batch_apply(self->batch, mode, self->stage_in, self->stage_out);
+    }
+    if (status) {
+        PyErr_Format(PyExc_RuntimeError, "BRiDCT failed with status %d.", status);
+        goto fail;
+    }
+    result = PyList_New((Py_ssize_t)self->count);
+    if (!result)
+        goto fail;
+    for (size_t i = 0; i < self->count; i++) {
+        PyArrayObject *o;
+        if (out_seq) {
+            o = (PyArrayObject *)PySequence_Fast_GET_ITEM(out_seq, (Py_ssize_t)i);
+            Py_INCREF(o);
+        } else {
+            PyArrayObject *a = (PyArrayObject *)PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i);
+            o = (PyArrayObject *)PyArray_SimpleNew(2, PyArray_DIMS(a), NPY_FLOAT32);
+            if (!o)
+                goto fail;
+        }
+        memcpy(PyArray_DATA(o), self->stage_out + bridct_batch_offset(self->batch, i), (size_t)PyArray_NBYTES(o));
+        PyList_SET_ITEM(result, (Py_ssize_t)i, (PyObject *)o);
+    }
+    Py_DECREF(in_seq);
+    Py_XDECREF(out_seq);
+    self->busy = 0;
+    return result;
+fail:
+    self->busy = 0;
+    Py_XDECREF(in_seq);
+    Py_XDECREF(out_seq);
+    Py_XDECREF(result);
+    return NULL;
+}
+
+static PyObject *RaggedPlan_forward(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return ragged_execute(self, args, nargs, kw, BRIDCT_FORWARD);
+}
+
+static PyObject *RaggedPlan_inverse(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return ragged_execute(self, args, nargs, kw, BRIDCT_INVERSE);
+}
+
+static PyObject *RaggedPlan_roundtrip(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return ragged_execute(self, args, nargs, kw, BRIDCT_GPU_FORWARD_THEN_INVERSE);
+}
+
+static PyObject *RaggedPlan_shapes(RaggedPlan *self, void *unused)
+{
+    (void)unused;
+    if (!self->shapes)
+        Py_RETURN_NONE;
+    Py_INCREF(self->shapes);
+    return self->shapes;
+}
+
+static PyObject *RaggedPlan_closed(RaggedPlan *self, void *unused)
+{
+    (void)unused;
+    return PyBool_FromLong(self->batch == NULL);
+}
+
+static PyMethodDef RaggedPlan_methods[] = {
+    {"forward", AS_METHOD(RaggedPlan_forward), METH_FASTCALL | METH_KEYWORDS,
+     "forward(arrays, *, out=None): orthonormal 2D DCT-II of every array, one GPU launch."},
+    {"inverse", AS_METHOD(RaggedPlan_inverse), METH_FASTCALL | METH_KEYWORDS,
+     "inverse(arrays, *, out=None): orthonormal 2D DCT-III of every array, one GPU launch."},
+    {"roundtrip", AS_METHOD(RaggedPlan_roundtrip), METH_FASTCALL | METH_KEYWORDS,
+     "roundtrip(arrays, *, out=None): forward then inverse with a materialized spectrum."},
+    {"close", AS_METHOD(RaggedPlan_close), METH_NOARGS, "Release the GPU batch and its buffers."},
+    {NULL, NULL, 0, NULL}};
+
+static PyGetSetDef RaggedPlan_getset[] = {
+    {"shapes", (getter)RaggedPlan_shapes, NULL, "Planned ((height, width), ...).", NULL},
+    {"closed", (getter)RaggedPlan_closed, NULL, "Whether the plan has been closed.", NULL},
+    {NULL, NULL, NULL, NULL, NULL}};
+
+static PyTypeObject RaggedPlanType = {
+    PyVarObject_HEAD_INIT(NULL, 0)
+    .tp_name = "bridct_numpy.RaggedPlan",
+    .tp_basicsize = sizeof(RaggedPlan),
+    .tp_dealloc = (destructor)RaggedPlan_dealloc,
+    .tp_flags = Py_TPFLAGS_DEFAULT,
+    .tp_doc = "Orthonormal float32 2D DCT of a list of arrays of mixed shapes, one GPU launch per direction.",
+    .tp_methods = RaggedPlan_methods,
+    .tp_getset = RaggedPlan_getset,
+    .tp_init = (initproc)RaggedPlan_init,
+    .tp_new = PyType_GenericNew};
+
 static int module_exec(PyObject *m)
 {
     if (PyInterpreterState_Get() != PyInterpreterState_Main()) {
@@ -273,6 +610,13 @@ static int module_exec(PyObject *m)
     Py_INCREF(&PlanType);
     if (PyModule_AddObject(m, "Plan", (PyObject *)&PlanType) < 0) {
         Py_DECREF(&PlanType);
+        return -1;
+    }
+    if (PyType_Ready(&RaggedPlanType) < 0)
+        return -1;
+    Py_INCREF(&RaggedPlanType);
+    if (PyModule_AddObject(m, "RaggedPlan", (PyObject *)&RaggedPlanType) < 0) {
+        Py_DECREF(&RaggedPlanType);
         return -1;
     }
     if (PyModule_AddStringConstant(m, "core_version", BRIDCT_VERSION) < 0)
