@@ -62,7 +62,8 @@ def snap(c):
 
 
 class Graph:
-    """Linear DAG.  ops[i] is ('in', k) or ('lin', a, b, r) meaning v[a] + r * v[b].
+    """Linear DAG.  ops[i] is ('in', k), ('lin', a, b, r) meaning v[a] + r * v[b], or
+    ('mul', a, f) meaning f * v[a].
 
     A handle is None (exact zero) or (c, i) meaning c * v[i]; c is a deferred factor.
     """
@@ -118,6 +119,24 @@ class Graph:
             return (snap(items[0][0]), items[0][1])
         return self._pair(items[0], items[1])
 
+    def mul(self, i, f):
+        """Node f * v[i]."""
+        key = ('mul', i, float('%.12e' % f))
+        nid = self.memo.get(key)
+        if nid is None:
+            nid = self._new(('mul', i, f))
+            self.memo[key] = nid
+        return nid
+
+    def lin(self, a, b, r):
+        """Node v[a] + r * v[b] (no factoring)."""
+        key = (a, b, float('%.12e' % r))
+        nid = self.memo.get(key)
+        if nid is None:
+            nid = self._new(('lin', a, b, r))
+            self.memo[key] = nid
+        return nid
+
     def live(self, outputs):
         seen = set()
         stack = [h[1] for h in outputs if h is not None]
@@ -129,11 +148,13 @@ class Graph:
             op = self.ops[i]
             if op[0] == 'lin':
                 stack.extend((op[1], op[2]))
+            elif op[0] == 'mul':
+                stack.append(op[1])
         return seen
 
     def counts(self, outputs):
         live = self.live(outputs)
-        adds = fmas = 0
+        adds = fmas = inner_muls = 0
         for i in live:
             op = self.ops[i]
             if op[0] == 'lin':
@@ -141,7 +162,9 @@ class Graph:
                     adds += 1
                 else:
                     fmas += 1
-        muls = sum(1 for h in outputs if h is not None and h[0] not in (1.0, -1.0))
+            elif op[0] == 'mul':
+                inner_muls += 1
+        muls = inner_muls + sum(1 for h in outputs if h is not None and h[0] not in (1.0, -1.0))
         return {'add': adds, 'fma': fmas, 'mul': muls,
                 'flops': adds + 2 * fmas + muls, 'instructions': adds + fmas + muls}
 
@@ -153,6 +176,8 @@ class Graph:
             op = self.ops[i]
             if op[0] == 'in':
                 v[i] = np.asarray(x[op[1]], dtype=dtype)
+            elif op[0] == 'mul':
+                v[i] = (np.float32(op[2]) * v[op[1]]).astype(np.float32) if dtype == np.float32 else op[2] * v[op[1]]
             else:
                 a, b, r = v[op[1]], v[op[2]], op[3]
                 if dtype == np.float32:
@@ -294,6 +319,9 @@ def transpose(g, outputs, n_in):
         if op[0] == 'in':
             result[op[1]] = a_h
             continue
+        if op[0] == 'mul':
+            adj.setdefault(op[1], []).append((op[2], a_h))
+            continue
         adj.setdefault(op[1], []).append((1.0, a_h))
         adj.setdefault(op[2], []).append((op[3], a_h))
     return g2, [result.get(i) for i in range(n_in)]
@@ -355,6 +383,8 @@ def emit_body(g, outs, in_names, out_names):
         name[i] = t
         if op[0] == 'in':
             lines.append('    const float %s = %s;' % (t, in_names[op[1]]))
+        elif op[0] == 'mul':
+            lines.append('    const float %s = %s * %s;' % (t, flit(op[2]), name[op[1]]))
         else:
             a, b, r = name[op[1]], name[op[2]], op[3]
             if r == 1.0:
@@ -387,6 +417,20 @@ def fp32_error(g, outs, n_in, M, trials=256, seed=1):
     return float(np.abs(y32 - ref).max() / np.abs(ref).max())
 
 
+KEEP_BELOW = 0.7
+
+
+def bound_output(g, h):
+    """Output handle (c, i) with a small pending factor c rebuilt as a node at scale c."""
+    c, i = h
+    if abs(c) >= KEEP_BELOW:
+        return h
+    op = g.ops[i]
+    if op[0] == 'lin':
+        return (1.0, g.lin(g.mul(op[1], c), op[2], snap(c * op[3])))
+    return (1.0, g.mul(i, c))
+
+
 def build_all(max_real=64, max_cplx=32, verbose=False):
     items = []   # (kind, n, sigma/inverse, g, outs, n_in, M)
     factors = {}
@@ -397,6 +441,11 @@ def build_all(max_real=64, max_cplx=32, verbose=False):
         check('sj_%d' % n, g, outs, n, F)
         # Fold the factors still pending at the outputs into the normalization:
         # the codelet computes D^-1 F_n, D = diag(d), and F_n^T = (D^-1 F_n)^T D.
+        # A pending factor below KEEP_BELOW would leave that output (the graph's largest
+        # value) larger than F_n's by 1/d, which costs range near the float32 maximum;
+        # such a factor is applied inside the graph instead, on the output's last node
+        # (f a + (f r) b: one extra multiply), so those outputs come out at F_n's scale.
+        outs = [bound_output(g, h) for h in outs]
         d = [h[0] for h in outs]
         raw = [(1.0, h[1]) for h in outs]
         M = F / np.array(d)[:, None]

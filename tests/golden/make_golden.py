@@ -18,6 +18,8 @@ library compiled from its own sources by oracle/Makefile (oracle/_ref/libbridct.
   input-relative Frobenius error against the FP64 Q-matrix closed form
   (dataset/evaluate.py:17-21), and the reference's published maximum over core cases
   (dataset/DATASET_CARD.md:34) for comparison.
+* corpus_manifest.json: id, size, domain and sha256 of every corpus array
+  (dataset/manifest.json), which pins the corpus generator of tests/corpus_gen.py.
 """
 import argparse
 import ctypes
@@ -101,6 +103,10 @@ def main():
         rows=rows)
     with open(os.path.join(HERE, 'corpus_summary.json'), 'w') as f:
         json.dump(summary, f, indent=0)
+    with open(os.path.join(HERE, 'corpus_manifest.json'), 'w') as f:
+        json.dump({'version': manifest['version'], 'archive_sha256': manifest['archive_sha256'],
+                   'cases': [dict(id=c['id'], N=c['N'], domain=c['domain'], sha256=c['sha256'])
+                             for c in manifest['cases']]}, f, indent=0)
     ids = sorted(small)
     np.savez_compressed(
         os.path.join(HERE, 'corpus_small.npz'),
