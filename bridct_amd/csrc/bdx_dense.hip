@@ -12,13 +12,15 @@
 // with Q_n' = s_n Q_n, s_n a power of two near sqrt(n) (so that the lo parts of the
 // matrix entries stay normal half-precision numbers); the final scale is exact.
 //
-// Kernel: 256 lanes, a 64 x 64 output tile per workgroup, four waves of 32 x 32, k steps
+// Kernel: 256 lanes, a 64 x 64 or 128 x 128 output tile per workgroup, four waves, k steps
 // of 32 staged in LDS as hi/lo halves ([row][k] for A, [column][k] for B, so that every
 // lane reads its eight k values of a v_mfma_f32_32x32x16 operand with one 16-byte load).
 // MFMA lane layout (checked on gfx950): lane l holds A[l % 32][8 (l / 32) + i],
 // B[8 (l / 32) + i][l % 32], and result element a at row 8 (a / 4) + 4 (l / 32) + a % 4,
 // column l % 32.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "bridct_device.h"
 
@@ -29,7 +31,7 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32, PITCH = BK + 8;   // pitch in half-precision elements
+constexpr int BK = 32, PITCH = BK + 8;   // k step; LDS row pitch in half-precision elements
 
 template <int PREC> struct Half;
 template <> struct Half<BDX_DENSE_F16> {
@@ -56,92 +58,140 @@ template <typename T> __device__ __forceinline__ void split(float x, T &hi, T &l
     lo = (T)(x - (float)hi);
 }
 
+// Eight consecutive k values of one row (A) or column (B) -> hi and lo halves, each
+// written with one 16-byte LDS store (row pitch 80 B: sixteen lanes of consecutive rows
+// hit sixteen different 4-bank groups).
+template <typename V> __device__ __forceinline__ void stage8(const float (&x)[8], V *hi, V *lo)
+{
+    V h, l;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        typename std::remove_reference<decltype(h[0])>::type a, b;
+        split(x[i], a, b);
+        h[i] = a;
+        l[i] = b;
+    }
+    *hi = h;
+    *lo = l;
+}
+
 // C[b] = scale * A[b] B[b]; A is M x K row-major; B is K x N row-major, or N x K row-major
-// when BT (i.e. B^T given).  M, N multiples of 64, K a multiple of 32; batch = gridDim.z.
-template <int PREC, bool BT>
-__global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, const float *__restrict__ B,
+// when BT (i.e. B^T given).  M, N multiples of TILE (64 or 128), K a multiple of 32;
+// batch = gridDim.z.  Four waves of (TILE/2) x (TILE/2), i.e. S x S MFMA tiles of 32 x 32
+// each (S = TILE / 64), so that every operand fragment read from LDS feeds S tiles.
+// Lane l stages k values 8 (l % 4) .. +7 of tile rows l / 4 + 64 j (A, B^T: two float4
+// loads each) or of tile columns l % 64 + 64 j (B: eight loads, each wave reading 64
+// consecutive floats of a row).
+template <int PREC, bool BT, int TILE>
+__global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *__restrict__ A, const float *__restrict__ B,
                                               float *__restrict__ C, int N, int K, long long sa, long long sb,
                                               long long sc, float scale)
 {
     using H = Half<PREC>;
     using T = typename H::T;
-    __shared__ __attribute__((aligned(16))) T a_hi[BM * PITCH], a_lo[BM * PITCH];
-    __shared__ __attribute__((aligned(16))) T b_hi[BN * PITCH], b_lo[BN * PITCH];
+    using V = typename H::V;
+    constexpr int S = TILE / 64;
+    __shared__ __attribute__((aligned(16))) T a_hi[TILE * PITCH], a_lo[TILE * PITCH];
+    __shared__ __attribute__((aligned(16))) T b_hi[TILE * PITCH], b_lo[TILE * PITCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    const long long m0 = (long long)blockIdx.y * TILE, n0 = (long long)blockIdx.x * TILE;
     A += blockIdx.z * sa + m0 * K;
     B += blockIdx.z * sb;
     C += blockIdx.z * sc;
-    f16v acc = {}, acc_lo = {};   // hi x hi products, and the two small cross products
+    f16v acc[S][S] = {}, acc_lo[S][S] = {};   // hi x hi products, and the two small cross products
+    const int ar = tid >> 2, ak = (tid & 3) * 8;            // A / B^T: row, first k
+    const int bn = tid & 63, bk = (tid >> 6) * 8;           // B: column, first k
     for (int k0 = 0; k0 < K; k0 += BK) {
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            // A tile: 64 rows x 32 floats, float4 per lane
-            const int idx = tid + 256 * j, r = idx >> 3, c = (idx & 7) * 4;
-            const float4 v = *reinterpret_cast<const float4 *>(A + (long long)r * K + k0 + c);
-            T *ph = a_hi + r * PITCH + c, *pl = a_lo + r * PITCH + c;
-            split(v.x, ph[0], pl[0]);
-            split(v.y, ph[1], pl[1]);
-            split(v.z, ph[2], pl[2]);
-            split(v.w, ph[3], pl[3]);
+        for (int j = 0; j < S; j++) {
+            float x[8];
+            const float *src = A + (long long)(ar + 64 * j) * K + k0 + ak;
+            const float4 u = *reinterpret_cast<const float4 *>(src);
+            const float4 v = *reinterpret_cast<const float4 *>(src + 4);
+            x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
+            const int o = (ar + 64 * j) * PITCH + ak;
+            stage8(x, reinterpret_cast<V *>(a_hi + o), reinterpret_cast<V *>(a_lo + o));
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int idx = tid + 256 * j;
+        for (int j = 0; j < S; j++) {
+            float x[8];
             if constexpr (BT) {
-                // B^T tile: 64 rows (n) x 32 floats (k)
-                const int r = idx >> 3, c = (idx & 7) * 4;
-                const float4 v = *reinterpret_cast<const float4 *>(B + (n0 + r) * K + k0 + c);
-                T *ph = b_hi + r * PITCH + c, *pl = b_lo + r * PITCH + c;
-                split(v.x, ph[0], pl[0]);
-                split(v.y, ph[1], pl[1]);
-                split(v.z, ph[2], pl[2]);
-                split(v.w, ph[3], pl[3]);
+                const float *src = B + (n0 + ar + 64 * j) * K + k0 + ak;
+                const float4 u = *reinterpret_cast<const float4 *>(src);
+                const float4 v = *reinterpret_cast<const float4 *>(src + 4);
+                x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
+                const int o = (ar + 64 * j) * PITCH + ak;
+                stage8(x, reinterpret_cast<V *>(b_hi + o), reinterpret_cast<V *>(b_lo + o));
             } else {
-                // B tile: 32 rows (k) x 64 floats (n), stored transposed
-                const int r = idx >> 4, c = (idx & 15) * 4;
-                const float4 v = *reinterpret_cast<const float4 *>(B + (long long)(k0 + r) * N + n0 + c);
-                split(v.x, b_hi[(c + 0) * PITCH + r], b_lo[(c + 0) * PITCH + r]);
-                split(v.y, b_hi[(c + 1) * PITCH + r], b_lo[(c + 1) * PITCH + r]);
-                split(v.z, b_hi[(c + 2) * PITCH + r], b_lo[(c + 2) * PITCH + r]);
-                split(v.w, b_hi[(c + 3) * PITCH + r], b_lo[(c + 3) * PITCH + r]);
+                const float *src = B + (long long)(k0 + bk) * N + n0 + bn + 64 * j;
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    x[i] = src[(long long)i * N];
+                const int o = (bn + 64 * j) * PITCH + bk;
+                stage8(x, reinterpret_cast<V *>(b_hi + o), reinterpret_cast<V *>(b_lo + o));
             }
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
-            const int ao = (wm * 32 + (lane & 31)) * PITCH + kk + 8 * (lane >> 5);
-            const int bo = (wn * 32 + (lane & 31)) * PITCH + kk + 8 * (lane >> 5);
-            const auto ah = *reinterpret_cast<const typename H::V *>(a_hi + ao);
-            const auto al = *reinterpret_cast<const typename H::V *>(a_lo + ao);
-            const auto bh = *reinterpret_cast<const typename H::V *>(b_hi + bo);
-            const auto bl = *reinterpret_cast<const typename H::V *>(b_lo + bo);
-            acc_lo = H::mfma(al, bh, acc_lo);
-            acc_lo = H::mfma(ah, bl, acc_lo);
-            acc = H::mfma(ah, bh, acc);
+            V ah[S], al[S], bh[S], bl[S];
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                const int ao = (wm * (TILE / 2) + 32 * i + (lane & 31)) * PITCH + kk + 8 * (lane >> 5);
+                const int bo = (wn * (TILE / 2) + 32 * i + (lane & 31)) * PITCH + kk + 8 * (lane >> 5);
+                ah[i] = *reinterpret_cast<const V *>(a_hi + ao);
+                al[i] = *reinterpret_cast<const V *>(a_lo + ao);
+                bh[i] = *reinterpret_cast<const V *>(b_hi + bo);
+                bl[i] = *reinterpret_cast<const V *>(b_lo + bo);
+            }
+#pragma unroll
+            for (int i = 0; i < S; i++)
+#pragma unroll
+                for (int j = 0; j < S; j++) {
+                    acc_lo[i][j] = H::mfma(al[i], bh[j], acc_lo[i][j]);
+                    acc_lo[i][j] = H::mfma(ah[i], bl[j], acc_lo[i][j]);
+                    acc[i][j] = H::mfma(ah[i], bh[j], acc[i][j]);
+                }
         }
         __syncthreads();
     }
-    const long long col = n0 + wn * 32 + (lane & 31);
 #pragma unroll
-    for (int a = 0; a < 16; a++) {
-        const long long row = m0 + wm * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
-        C[row * N + col] = (acc[a] + acc_lo[a]) * scale;
-    }
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const long long col = n0 + wn * (TILE / 2) + 32 * j + (lane & 31);
+#pragma unroll
+            for (int a = 0; a < 16; a++) {
+                const long long row = m0 + wm * (TILE / 2) + 32 * i + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
+                C[row * N + col] = (acc[i][j][a] + acc_lo[i][j][a]) * scale;
+            }
+        }
+}
+
+template <int PREC, int TILE>
+void launch(bool bt, dim3 grid, hipStream_t s, const float *A, const float *B, float *C, int N, int K, long long sa,
+            long long sb, long long sc, float scale)
+{
+    if (bt)
+        hipLaunchKernelGGL((k_gemm<PREC, true, TILE>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale);
+    else
+        hipLaunchKernelGGL((k_gemm<PREC, false, TILE>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale);
 }
 
 template <int PREC>
 int gemm(bool bt, const float *A, const float *B, float *C, int M, int N, int K, int batch, long long sa,
          long long sb, long long sc, float scale, hipStream_t s)
 {
-    if (M % BM || N % BN || K % BK || batch < 1 || batch > 65535 || M / BM > 65535 || N / BN > 65535)
+    // 128 x 128 tiles where both dimensions allow, else 64 x 64
+    const int tile = M % 128 == 0 && N % 128 == 0 ? 128 : 64;
+    if (M % 64 || N % 64 || K % BK || batch < 1 || batch > 65535 || M / tile > 65535 || N / tile > 65535)
         return -1;
-    const dim3 grid((unsigned)(N / BN), (unsigned)(M / BM), (unsigned)batch), block(256);
-    if (bt)
-        hipLaunchKernelGGL((k_gemm<PREC, true>), grid, block, 0, s, A, B, C, N, K, sa, sb, sc, scale);
+    const dim3 grid((unsigned)(N / tile), (unsigned)(M / tile), (unsigned)batch);
+    if (tile == 128)
+        launch<PREC, 128>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale);
     else
-        hipLaunchKernelGGL((k_gemm<PREC, false>), grid, block, 0, s, A, B, C, N, K, sa, sb, sc, scale);
+        launch<PREC, 64>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -31,8 +31,15 @@ int launch_hw(const float *in, float *out, size_t batch, int pack, int mode, con
         constexpr int P = W + 1;
         constexpr int per_max = kSmallLds / (H * P);
         // automatic (pack 0): enough workgroups to cover the GPU several times before
-        // packing arrays together; otherwise the requested arrays per workgroup
-        size_t per = pack > 0 ? (size_t)pack : (batch + 1023) / 1024;
+        // packing arrays together; otherwise the requested arrays per workgroup (tuned on
+        // large batches), reduced for small batches so that about 512 workgroups remain
+        size_t per = (batch + 1023) / 1024;
+        if (pack > 0) {
+            per = (size_t)pack;
+            const size_t keep = (batch + 511) / 512;
+            if (per > keep)
+                per = keep;
+        }
         if (per < 1)
             per = 1;
         if (per > (size_t)per_max)

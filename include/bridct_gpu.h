@@ -40,7 +40,8 @@ BRIDCT_API int bridct_plan_set_route(bridct_plan *plan, int route);
 BRIDCT_API int bridct_plan_get_route(const bridct_plan *plan);
 
 /* Arrays per workgroup of the whole-array kernel (shapes whose array fits one
- * workgroup's LDS): 0 = automatic, else 1..4096 (clamped to what fits). */
+ * workgroup's LDS): 0 = automatic, else 1..4096 (clamped to what fits, and lowered for
+ * batches too small to give about 512 workgroups at that packing). */
 BRIDCT_API int bridct_plan_set_pack(bridct_plan *plan, int pack);
 BRIDCT_API int bridct_plan_get_pack(const bridct_plan *plan);
 
