@@ -79,7 +79,7 @@ template <int W, int OP>
 __device__ __noinline__ void row_w(const float *in, float *out, size_t row0, int nrows, float *smem,
                                    const float *__restrict__ tab)
 {
-    static_assert(row_unit_lds<W, OP, TH>() <= BDX_BATCH_LDS && row_unit_rows<W, TH>() == BDX_ROW_UNIT(W),
+    static_assert(row_unit_lds<W, OP, TH>() <= BDX_BATCH_LDS && row_unit_rows<W, TH>() == BDX_ROW_BAND(W),
                   "row unit geometry");
     row_unit<W, OP, TH>(in, out, row0, nrows, smem, tab, false);
 }
@@ -104,7 +104,7 @@ template <int H>
 __device__ __noinline__ void col_h(const float *in, float *out, int w, size_t base, int c0, int ncols, float *smem,
                                    const float *__restrict__ tab)
 {
-    static_assert(col_unit_lds<H, TH>() <= BDX_BATCH_LDS && col_unit_cols<H, TH>() == BDX_COL_UNIT(H),
+    static_assert(col_unit_lds<H, TH>() <= BDX_BATCH_LDS && col_unit_cols<H, TH>() == BDX_COL_STRIP(H),
                   "column unit geometry");
     col_unit<H, BDX_MODE, TH>(in, out, w, base, c0, ncols, smem, tab, false);
 }
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, co
             s_item = __hip_atomic_fetch_add(ctr + BDX_CTR_HEAD, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const unsigned i = s_item;
-        __syncthreads();
+        __syncthreads();   // every lane has read s_item before thread 0 writes it again
         if (i >= nitems)
             break;
         const bdx_item it = items[i];
@@ -183,19 +183,31 @@ __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, co
         if (kind == BDX_ITEM_SMALL) {
             small_any(h, w, in, out, base, (int)it.unit, smem, tab);
         } else if (kind == BDX_ITEM_ROW) {
-            const int r0 = (int)it.unit * BDX_ROW_UNIT(w);
-            row_any<ROW_OP>(w, in + base, out + base, (size_t)r0, min(BDX_ROW_UNIT(w), h - r0), smem, tab);
+            const int r0 = (int)it.unit * BDX_ROW_UNIT(w), r1 = min(r0 + BDX_ROW_UNIT(w), h);
+            for (int r = r0; r < r1; r += BDX_ROW_BAND(w)) {
+                if (r > r0)
+                    __syncthreads();   // the LDS of the previous band is free
+                row_any<ROW_OP>(w, in + base, out + base, (size_t)r, min(BDX_ROW_BAND(w), r1 - r), smem, tab);
+            }
             signal_done(rows_done);
         } else if (kind == BDX_ITEM_COL) {
             wait_for(rows_done, (unsigned)BDX_CEIL(h, BDX_ROW_UNIT(w)), err, spin_limit);
-            const int c0 = (int)it.unit * BDX_COL_UNIT(h);
-            col_any(h, w, out, out, base, c0, min(BDX_COL_UNIT(h), w - c0), smem, tab);
+            const int c0 = (int)it.unit * BDX_COL_UNIT(h), c1 = min(c0 + BDX_COL_UNIT(h), w);
+            for (int cc = c0; cc < c1; cc += BDX_COL_STRIP(h)) {
+                if (cc > c0)
+                    __syncthreads();
+                col_any(h, w, out, out, base, cc, min(BDX_COL_STRIP(h), c1 - cc), smem, tab);
+            }
             if (BDX_MODE == 2)
                 signal_done(cols_done);
         } else if constexpr (BDX_MODE == 2) {
             wait_for(cols_done, (unsigned)BDX_CEIL(w, BDX_COL_UNIT(h)), err, spin_limit);
-            const int r0 = (int)it.unit * BDX_ROW_UNIT(w);
-            row_any<I0>(w, out + base, out + base, (size_t)r0, min(BDX_ROW_UNIT(w), h - r0), smem, tab);
+            const int r0 = (int)it.unit * BDX_ROW_UNIT(w), r1 = min(r0 + BDX_ROW_UNIT(w), h);
+            for (int r = r0; r < r1; r += BDX_ROW_BAND(w)) {
+                if (r > r0)
+                    __syncthreads();
+                row_any<I0>(w, out + base, out + base, (size_t)r, min(BDX_ROW_BAND(w), r1 - r), smem, tab);
+            }
         }
         __syncthreads();   // LDS is reused by the next item
     }

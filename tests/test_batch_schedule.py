@@ -37,12 +37,15 @@ def team(n):
     return 1 if n <= 64 else 16 if n >= 1024 else 8 if n >= 256 else 4
 
 
+BANDS = 1   # BDX_ITEM_BANDS: bands / strips per item for lines of 512 and more
+
+
 def row_unit(w):
-    return THREADS // team(w) if w >= 128 else THREADS
+    return (THREADS // team(w) if w >= 128 else THREADS) * (BANDS if w >= 512 else 1)
 
 
 def col_unit(h):
-    return THREADS // team(h) if h >= 128 else THREADS
+    return (THREADS // team(h) if h >= 128 else THREADS) * (BANDS if h >= 512 else 1)
 
 
 def is_small(h, w):
@@ -130,10 +133,11 @@ def check(shapes, lag):
     return s
 
 
-def simulate(s, fused, blocks, seed):
+def simulate(s, fused, blocks, seed, prefetch=True):
     """Persistent grid of `blocks` workers pulling items in queue order; an item that waits
-    holds its worker until its dependencies are done.  Returns the makespan (or raises on
-    a deadlock)."""
+    holds its worker until its dependencies are done.  With prefetch, a worker claims its
+    next item when it starts the current one and holds it unstarted (a claiming order the
+    schedule must also survive).  Returns the makespan (or raises on a deadlock)."""
     items = s['items'][fused]
     need = {}
     for run, idx, unit, kind, ctr in items:
@@ -142,8 +146,9 @@ def simulate(s, fused, blocks, seed):
     done = {}
     rng = random.Random(seed)
     t, head = 0.0, 0
-    waiting = []      # (worker, item) blocked on a dependency
+    waiting = []      # (worker, item) started but blocked on a dependency
     running = []      # heap of (finish time, worker, item)
+    held = {}         # worker -> claimed, not started item
     free = list(range(blocks))
 
     def ready(it):
@@ -154,11 +159,22 @@ def simulate(s, fused, blocks, seed):
             return done.get((ctr, COL), 0) == need[(ctr, COL)]
         return True
 
-    while head < len(items) or waiting or running:
-        while free and head < len(items):
-            wkr = free.pop()
-            waiting.append((wkr, items[head]))
-            head += 1
+    while head < len(items) or waiting or running or held:
+        idle = []
+        for wkr in free:
+            if wkr in held:
+                cur = held.pop(wkr)
+            elif head < len(items):
+                cur = items[head]
+                head += 1
+            else:
+                idle.append(wkr)
+                continue
+            waiting.append((wkr, cur))
+            if prefetch and head < len(items):
+                held[wkr] = items[head]
+                head += 1
+        free = idle
         still = []
         for wkr, it in waiting:
             if ready(it):
@@ -189,6 +205,7 @@ def test_ragged_batches_and_lags():
         for fused in (0, 1):
             for blocks in (1, 3, 512):
                 simulate(s, fused, blocks, seed=blocks + lag)
+                simulate(s, fused, blocks, seed=blocks + lag, prefetch=False)
 
 
 def test_all_shapes_once():
