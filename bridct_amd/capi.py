@@ -53,6 +53,7 @@ SIGNATURES = {
     'bridct_batch_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_batch_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     'bridct_batch_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
+    'bridct_batch_set_spin_limit': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     'bridct_gpu_malloc': (ctypes.c_void_p, [ctypes.c_size_t]),
     'bridct_gpu_free': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_gpu_upload': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
@@ -204,6 +205,9 @@ class Batch:
 
     def synchronize(self):
         return self.lib.bridct_batch_synchronize(self.ptr)
+
+    def set_spin_limit(self, limit):
+        return self.lib.bridct_batch_set_spin_limit(self.ptr, limit)
 
     @property
     def stream(self):

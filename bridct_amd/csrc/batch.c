@@ -41,6 +41,7 @@ typedef struct {
 
 struct bridct_batch {
     size_t count, elements;
+    unsigned spin_limit;     /* dependency-wait bound of the persistent kernel */
     size_t *offsets;         /* element offset of each array */
     unsigned nruns, ntwo;    /* runs, two-pass arrays */
     unsigned blocks;
@@ -269,6 +270,7 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         return NULL;
     /* two 256-lane workgroups per CU (LDS and registers) */
     b->blocks = 2u * (unsigned)cus;
+    b->spin_limit = SPIN_LIMIT;
     bdx_schedule sc;
     if (bdx_batch_schedule(count, heights, widths, 3 * b->blocks, &sc)) {
         free(b);
@@ -326,7 +328,7 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
     const int fused = mode == BDX_ROUNDTRIP;
     return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_ctr, b->nctr,
-                            mode, b->d_tables, b->blocks, SPIN_LIMIT, b->stream);
+                            mode, b->d_tables, b->blocks, b->spin_limit, b->stream);
 }
 
 static int issue(bridct_batch *b, int mode, const float *d_in, float *d_out)
@@ -417,6 +419,20 @@ int bridct_batch_apply(bridct_batch *b, int mode, const float *input, float *out
         bdx_copy_d2h(output, b->d_out, bytes, b->stream))
         return BRIDCT_DEVICE_ERROR;
     return bridct_batch_synchronize(b);
+}
+
+int bridct_batch_set_spin_limit(bridct_batch *b, unsigned limit)
+{
+    if (!b || limit == 0)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (b->graph) {   /* the captured launches carry the old bound */
+        if (bdx_stream_sync(b->stream))
+            return BRIDCT_DEVICE_ERROR;
+        bdx_graph_destroy(b->graph);
+        b->graph = NULL;
+    }
+    b->spin_limit = limit;
+    return BRIDCT_OK;
 }
 
 void *bridct_batch_stream(const bridct_batch *b)

@@ -108,6 +108,11 @@ BRIDCT_API int bridct_batch_synchronize(bridct_batch *batch);
 /* Host buffers of bridct_batch_elements floats each (disjoint); synchronous. */
 BRIDCT_API int bridct_batch_apply(bridct_batch *batch, int mode, const float *input, float *output);
 BRIDCT_API void *bridct_batch_stream(const bridct_batch *batch);
+/* Bound on each dependency wait of the persistent kernel, in sleeps of the waiting
+ * workgroup (default 2^21, about a second).  A wait that runs out sets the batch's error
+ * flag (reported by bridct_batch_synchronize) and proceeds, so a call never hangs; a
+ * small bound makes that path testable.  Takes effect from the next call. */
+BRIDCT_API int bridct_batch_set_spin_limit(bridct_batch *batch, unsigned limit);
 
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);

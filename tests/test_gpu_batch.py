@@ -187,3 +187,33 @@ def test_batch_argument_errors():
         assert b.synchronize() == 0
     finally:
         b.close()
+
+
+def test_wait_bound_reports_error_and_recovers():
+    """A dependency wait that runs out sets the error flag instead of hanging: with a bound
+    of one sleep, the column items of a large array (queued right behind its rows) cannot
+    all find the rows done, synchronize reports BRIDCT_DEVICE_ERROR, and the next call with
+    the normal bound is correct again (the flag is reset by each launch)."""
+    shapes = [(1024, 1024)]
+    x = flat(ragged(shapes, 4))
+    b = capi.Batch(shapes)
+    d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    try:
+        d_in.upload(x)
+        assert b.set_spin_limit(0) == capi.INVALID_ARGUMENT
+        assert b.set_spin_limit(1) == 0
+        print('tiny wait bound', flush=True)
+        assert b.apply_device(0, d_in.ptr, d_out.ptr) == 0
+        assert b.synchronize() == capi.DEVICE_ERROR
+        assert b.set_spin_limit(1 << 21) == 0
+        assert b.apply_device(0, d_in.ptr, d_out.ptr) == 0
+        assert b.synchronize() == 0
+        y = np.empty_like(x)
+        d_out.download(y)
+        p = capi.Plan(1024, 1024)
+        r = np.empty_like(x)
+        assert p.apply_batch(0, x, r) == 0
+        p.close()
+        assert np.abs(y - r).max() <= 1e-6 * np.abs(r).max()
+    finally:
+        b.close()
