@@ -45,6 +45,7 @@ struct bridct_batch {
     size_t *offsets;         /* element offset of each array */
     unsigned nruns, ntwo;    /* runs, two-pass arrays */
     unsigned blocks;
+    int tiny;                /* all arrays <= BDX_TINY_MAX on both axes: tiny kernel */
     void *stream;
     float *d_tables;
     bdx_run *d_runs;
@@ -110,7 +111,7 @@ static int push_units(item_list *l, const two_pass *a, unsigned c, unsigned kind
 
 /* Builds the queue of one mode family (fused = round trip with ROW2 items). */
 static int build_items(item_list *l, const bdx_run *runs, unsigned nruns, const two_pass *tp,
-                       unsigned ntwo, int fused, unsigned lag)
+                       unsigned ntwo, int fused, unsigned lag, int tiny)
 {
     /* end position (queue index after the last item) of each array's previous stage */
     unsigned *row_end = malloc((ntwo + 1) * sizeof(unsigned));
@@ -141,7 +142,7 @@ static int build_items(item_list *l, const bdx_run *runs, unsigned nruns, const 
         const int h = (int)runs[r].h, w = (int)runs[r].w;
         if (!BDX_BATCH_SMALL(h, w))
             continue;
-        const unsigned per = (unsigned)BDX_BATCH_PER(h, w), n = runs[r].count;
+        const unsigned per = (unsigned)(tiny ? BDX_TINY_PER(h, w) : BDX_BATCH_PER(h, w)), n = runs[r].count;
         const unsigned parts = (n + per - 1) / per;
         for (unsigned p = 0, first = 0; p < parts && !rc; p++) {
             const unsigned take = (n - first) / (parts - p) + ((n - first) % (parts - p) != 0);
@@ -199,6 +200,14 @@ void bdx_schedule_free(bdx_schedule *s)
     memset(s, 0, sizeof(*s));
 }
 
+int bdx_batch_is_tiny(size_t count, const int *heights, const int *widths)
+{
+    for (size_t i = 0; i < count; i++)
+        if (heights[i] > BDX_TINY_MAX || widths[i] > BDX_TINY_MAX)
+            return 0;
+    return count > 0;
+}
+
 int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s)
 {
     memset(s, 0, sizeof(*s));
@@ -243,8 +252,9 @@ int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsi
     s->count = count;
     s->elements = off;
     qsort(tp, s->ntwo, sizeof(two_pass), by_size_desc);
-    const int rc = build_items(&lists[0], s->runs, s->nruns, tp, s->ntwo, 0, lag) ||
-                   build_items(&lists[1], s->runs, s->nruns, tp, s->ntwo, 1, lag);
+    s->tiny = bdx_batch_is_tiny(count, heights, widths);
+    const int rc = build_items(&lists[0], s->runs, s->nruns, tp, s->ntwo, 0, lag, s->tiny) ||
+                   build_items(&lists[1], s->runs, s->nruns, tp, s->ntwo, 1, lag, s->tiny);
     free(tp);
     s->items[0] = lists[0].items;
     s->items[1] = lists[1].items;
@@ -268,7 +278,8 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     bridct_batch *b = calloc(1, sizeof(*b));
     if (!b)
         return NULL;
-    /* two 256-lane workgroups per CU (LDS and registers) */
+    /* two 256-lane workgroups per CU (LDS and registers); the tiny kernel launches one
+     * workgroup per item instead */
     b->blocks = 2u * (unsigned)cus;
     b->spin_limit = SPIN_LIMIT;
     bdx_schedule sc;
@@ -285,8 +296,10 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     b->nitems[0] = sc.nitems[0];
     b->nitems[1] = sc.nitems[1];
     b->nctr = sc.nctr;
+    b->tiny = sc.tiny;
     float *host_tables = malloc(BDX_TABLE_FLOATS * sizeof(float));
-    if (!host_tables)
+    unsigned *zeros = calloc(b->nctr, sizeof(unsigned));   /* counters start at zero */
+    if (!host_tables || !zeros)
         goto fail;
     bridct_build_tables(host_tables);
     if (bdx_stream_create(&b->stream) ||
@@ -294,14 +307,16 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         upload((void **)&b->d_runs, sc.runs, sc.nruns * sizeof(bdx_run), b->stream) ||
         upload((void **)&b->d_items[0], sc.items[0], sc.nitems[0] * sizeof(bdx_item), b->stream) ||
         upload((void **)&b->d_items[1], sc.items[1], sc.nitems[1] * sizeof(bdx_item), b->stream) ||
-        bdx_malloc((void **)&b->d_ctr, b->nctr * sizeof(unsigned)) || bdx_stream_sync(b->stream))
+        upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) || bdx_stream_sync(b->stream))
         goto fail;
     free(host_tables);
+    free(zeros);
     bdx_schedule_free(&sc);
     return b;
 
 fail:
     free(host_tables);
+    free(zeros);
     bdx_schedule_free(&sc);
     bridct_batch_destroy(b);
     return NULL;
@@ -328,7 +343,7 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
     const int fused = mode == BDX_ROUNDTRIP;
     return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_ctr, b->nctr,
-                            mode, b->d_tables, b->blocks, b->spin_limit, b->stream);
+                            mode, b->d_tables, b->blocks, b->spin_limit, b->tiny, b->stream);
 }
 
 static int issue(bridct_batch *b, int mode, const float *d_in, float *d_out)

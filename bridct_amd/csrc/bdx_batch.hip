@@ -213,16 +213,53 @@ __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, co
     }
 }
 
+// ---- tiny kernel: whole arrays with both axes <= BDX_TINY_MAX only -----------------------
+
+template <int H> __device__ __forceinline__ void tiny_h(int w, const float *in, float *out, size_t off, int count,
+                                                        float *smem, const float *__restrict__ tab)
+{
+    switch (w) {
+    case 8: small_unit<H, 8, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
+    case 16: small_unit<H, 16, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
+    default: small_unit<H, 32, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
+    }
+}
+
+// Every item of a tiny batch is a SMALL item (the host checked the shapes), and no item
+// depends on another: one workgroup per item, in a plain grid the hardware schedules, with
+// the units inlined and a small LDS area, so that many workgroups stay resident per CU.
+__global__ __launch_bounds__(TH) void k_batch_tiny(const float *in, float *out, const bdx_run *__restrict__ runs,
+                                                   const bdx_item *__restrict__ items, const float *__restrict__ tab)
+{
+    static_assert(BDX_TINY_PER(BDX_TINY_MAX, BDX_TINY_MAX) >= 1, "a tiny array must fit one item");
+    static_assert(BDX_TINY_PER(8, 8) * 8 * 9 <= BDX_TINY_LDS && BDX_TINY_PER(32, 32) * 32 * 33 <= BDX_TINY_LDS,
+                  "a full tiny item must fit the LDS area");
+    __shared__ float smem[BDX_TINY_LDS];
+    const bdx_item it = items[blockIdx.x];
+    const bdx_run rn = runs[it.run];
+    const int h = (int)rn.h, w = (int)rn.w;
+    const size_t base = rn.off + (size_t)it.idx * h * w;
+    switch (h) {
+    case 8: tiny_h<8>(w, in, out, base, (int)it.unit, smem, tab); break;
+    case 16: tiny_h<16>(w, in, out, base, (int)it.unit, smem, tab); break;
+    default: tiny_h<32>(w, in, out, base, (int)it.unit, smem, tab); break;
+    }
+}
+
 }  // namespace
 
 #define BDX_CAT2(a, b) a##b
 #define BDX_CAT(a, b) BDX_CAT2(a, b)
 
-int BDX_CAT(launch_batch_m, BDX_MODE)(const float *in, float *out, const bdx_run *runs, const bdx_item *items,
-                                      unsigned nitems, unsigned *ctr, const float *tab, unsigned blocks,
-                                      unsigned spin_limit, hipStream_t s)
+int BDX_CAT(launch_batch_m, BDX_MODE)(int tiny, const float *in, float *out, const bdx_run *runs,
+                                      const bdx_item *items, unsigned nitems, unsigned *ctr, const float *tab,
+                                      unsigned blocks, unsigned spin_limit, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, nitems, ctr, tab, spin_limit);
+    if (tiny)
+        hipLaunchKernelGGL(k_batch_tiny, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
+    else
+        hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, nitems, ctr, tab,
+                           spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

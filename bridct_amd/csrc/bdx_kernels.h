@@ -50,11 +50,11 @@ int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mod
                 hipStream_t s);
 
 // Persistent batch kernel, one per mode (bdx_batch.hip).
-int launch_batch_m0(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+int launch_batch_m0(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
-int launch_batch_m1(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+int launch_batch_m1(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
-int launch_batch_m2(const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+int launch_batch_m2(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
 
 }  // namespace bdx

@@ -110,20 +110,22 @@ int bdx_device_cus(void)
 
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
                      unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, void *stream)
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, int tiny, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    // a fill kernel, not a memset node, so that the sequence can be captured in a graph
-    hipLaunchKernelGGL(k_zero, dim3((ncounters + 255) / 256), dim3(256), 0, s, d_counters, ncounters);
+    // a fill kernel, not a memset node, so that the sequence can be captured in a graph; the
+    // tiny kernel uses no counters (the caller zeroed them once, for the error flag)
+    if (!tiny)
+        hipLaunchKernelGGL(k_zero, dim3((ncounters + 255) / 256), dim3(256), 0, s, d_counters, ncounters);
     if (hipGetLastError() != hipSuccess || nitems == 0)
         return nitems == 0 ? 0 : -1;
     switch (mode) {
     case BDX_FORWARD:
-        return bdx::launch_batch_m0(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m0(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     case BDX_INVERSE:
-        return bdx::launch_batch_m1(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m1(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     case BDX_ROUNDTRIP:
-        return bdx::launch_batch_m2(d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m2(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     default:
         return -1;
     }

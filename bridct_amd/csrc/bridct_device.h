@@ -43,6 +43,14 @@ extern "C" {
 #define BDX_ROW_UNIT(w) (BDX_ROW_BAND(w) * ((w) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_COL_UNIT(h) (BDX_COL_STRIP(h) * ((h) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_CEIL(a, b) (((a) + (b) - 1) / (b))
+/* Batches whose arrays all have both axes at most BDX_TINY_MAX have no dependent items: they
+ * run the tiny kernel instead of the persistent queue, one workgroup per item, about
+ * BDX_TINY_PIXELS pixels per item (the whole-array packing that measured fastest for these
+ * shapes) and BDX_TINY_LDS floats of LDS, so that many workgroups stay resident per CU. */
+#define BDX_TINY_MAX 32
+#define BDX_TINY_PIXELS 2048
+#define BDX_TINY_LDS 2304      /* BDX_TINY_PIXELS * (w + 1) / w floats at w = 8 */
+#define BDX_TINY_PER(h, w) (BDX_TINY_PIXELS / ((h) * (w)))
 /* work item kinds */
 #define BDX_ITEM_SMALL 0u
 #define BDX_ITEM_ROW 1u
@@ -71,7 +79,7 @@ typedef struct {
  * wait (a wait that runs out sets the error flag instead of hanging). */
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
                      unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, void *stream);
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, int tiny, void *stream);
 int bdx_device_cus(void);
 
 /* Host-side schedule of a batch (no device needed): runs of consecutive equal shapes,
@@ -86,8 +94,12 @@ typedef struct {
     bdx_item *items[2];
     unsigned nitems[2];
     unsigned nctr;
+    int tiny;                 /* every array fits the tiny kernel; SMALL items hold BDX_TINY_PER */
 } bdx_schedule;
 int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s);
+/* 1 when every array of the shape list fits the tiny kernel (both axes <= BDX_TINY_MAX);
+ * its schedule then packs BDX_TINY_PER(h, w) arrays per item at most. */
+int bdx_batch_is_tiny(size_t count, const int *heights, const int *widths);
 void bdx_schedule_free(bdx_schedule *s);
 
 /* Graph capture of a sequence of calls issued by `fn(arg)` on `stream`. */

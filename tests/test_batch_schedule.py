@@ -30,7 +30,8 @@ class Schedule(ctypes.Structure):
     _fields_ = [('count', ctypes.c_size_t), ('elements', ctypes.c_size_t),
                 ('offsets', ctypes.POINTER(ctypes.c_size_t)), ('runs', ctypes.POINTER(Run)),
                 ('nruns', ctypes.c_uint), ('ntwo', ctypes.c_uint),
-                ('items', ctypes.POINTER(Item) * 2), ('nitems', ctypes.c_uint * 2), ('nctr', ctypes.c_uint)]
+                ('items', ctypes.POINTER(Item) * 2), ('nitems', ctypes.c_uint * 2), ('nctr', ctypes.c_uint),
+                ('tiny', ctypes.c_int)]
 
 
 def team(n):
@@ -52,8 +53,14 @@ def is_small(h, w):
     return h <= 128 and w <= 128
 
 
-def per_unit(h, w):
-    return 20400 // (h * (w + 1))
+def is_tiny(shapes):
+    return all(h <= 32 and w <= 32 for h, w in shapes)
+
+
+def per_unit(h, w, tiny=False):
+    # tiny batches (every array <= 32 x 32): about 2048 pixels per item; otherwise as many
+    # whole arrays as fit the persistent kernel's LDS area
+    return 2048 // (h * w) if tiny else 20400 // (h * (w + 1))
 
 
 def ceil(a, b):
@@ -76,7 +83,7 @@ def schedule(shapes, lag):
         'elements': s.elements,
         'offsets': [s.offsets[i] for i in range(s.count)],
         'runs': [(r.h, r.w, r.count, r.off) for r in (s.runs[i] for i in range(s.nruns))],
-        'ntwo': s.ntwo, 'nctr': s.nctr,
+        'ntwo': s.ntwo, 'nctr': s.nctr, 'tiny': s.tiny,
         'items': [[(it.run, it.idx, it.unit, it.kind_ctr >> 28, it.kind_ctr & 0x0FFFFFFF)
                    for it in (s.items[q][i] for i in range(s.nitems[q]))] for q in (0, 1)],
     }
@@ -101,6 +108,7 @@ def check(shapes, lag):
     assert len(arrays) == len(shapes)
     two = [a for a, (h, w) in zip(arrays, shapes) if not is_small(h, w)]
     assert s['ntwo'] == len(two) and s['nctr'] == 2 + 2 * len(two)
+    assert s['tiny'] == is_tiny(shapes)
     for fused in (0, 1):
         items = s['items'][fused]
         small_seen = {}
@@ -109,7 +117,7 @@ def check(shapes, lag):
         for pos, (run, idx, unit, kind, ctr) in enumerate(items):
             h, w = s['runs'][run][:2]
             if kind == SMALL:
-                assert is_small(h, w) and 1 <= unit <= per_unit(h, w)
+                assert is_small(h, w) and 1 <= unit <= per_unit(h, w, s['tiny'])
                 for a in range(idx, idx + unit):
                     assert a < s['runs'][run][2]
                     small_seen[(run, a)] = small_seen.get((run, a), 0) + 1
@@ -217,11 +225,14 @@ def test_all_shapes_once():
 
 
 def test_many_small_blocks_split_evenly():
-    s = check([(8, 8)] * 100000, 256)
-    per = per_unit(8, 8)
-    counts = [u for _, _, u, k, _ in s['items'][0] if k == SMALL]
-    assert sum(counts) == 100000 and len(counts) == ceil(100000, per)
-    assert max(counts) - min(counts) <= 1
+    # all arrays <= 32 x 32: the tiny kernel's smaller LDS area sets the packing
+    for shapes in ([(8, 8)] * 100000, [(8, 8)] * 99999 + [(64, 8)]):
+        s = check(shapes, 256)
+        per = per_unit(8, 8, s['tiny'])
+        counts = [u for r, _, u, k, _ in s['items'][0] if k == SMALL and s['runs'][r][:2] == (8, 8)]
+        assert sum(counts) == 99999 + (shapes[-1] == (8, 8)) and len(counts) == ceil(sum(counts), per)
+        assert max(counts) - min(counts) <= 1
+    assert check([(32, 32), (8, 16)], 0)['tiny'] and not check([(32, 32), (8, 64)], 0)['tiny']
 
 
 @pytest.mark.parametrize('shapes', [[], [(8, 12)], [(2048, 8)], [(8, 8), (4, 8)]])

@@ -111,6 +111,28 @@ def test_uniform_small_blocks():
         p.close()
 
 
+def test_tiny_kernel_mix():
+    """Every array at most 32 x 32: the batch runs the tiny persistent kernel (smaller LDS
+    area, more resident workgroups); results equal the plan path, eager and replayed."""
+    rng = np.random.default_rng(13)
+    lengths = (8, 16, 32)
+    shapes = [(int(rng.choice(lengths)), int(rng.choice(lengths))) for _ in range(5000)]
+    arrays = ragged(shapes, 13)
+    b = capi.Batch(shapes)
+    try:
+        x = flat(arrays)
+        for mode in (0, 1, 2, FTI):
+            ref = plan_results(shapes, arrays, mode) if mode != FTI else arrays
+            for graph in (False, True, True):
+                print('tiny mix of 5000 arrays, mode', mode, 'graph', graph, flush=True)
+                y = b.split(run_device(b, mode, x, graph=graph))
+                for (h, w), yi, ri in zip(shapes, y, ref):
+                    tol = 1e-6 if mode != FTI else 2e-6
+                    assert np.abs(yi - ri).max() <= tol * np.abs(ri).max(), (h, w)
+    finally:
+        b.close()
+
+
 def test_large_arrays_with_small_fillers():
     shapes = [(1024, 1024)] * 3 + [(16, 16)] * 500 + [(512, 1024), (8, 1024), (1024, 8)]
     arrays = ragged(shapes, 5)
