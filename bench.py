@@ -12,7 +12,8 @@ copy as the practical bandwidth reference, and the reference CPU path timed on t
 cores.  Secondary legs (not part of `value`): the dense split-MFMA route vs the
 butterflies at 64^2, 256^2, 1024^2 (batch 128), 1,048,576 independent 8x8 blocks in one
 persistent launch vs per-block launches, and a ragged batch of 10,000 arrays of random
-supported shapes (seed 7) in one persistent launch vs one plan launch per array.
+supported shapes (seed 7) in one persistent launch vs one plan launch per array, and the
+fused 8x8 block codec step (u8 image -> quantized int16 coefficients, and back).
 """
 import argparse
 import ctypes
@@ -346,6 +347,47 @@ def leg_dense(hip):
     return res
 
 
+# Luminance quantization table of ITU-T T.81 Annex K (Table K.1), row-major u * 8 + v.
+JPEG_LUMA = np.array([
+    16, 11, 10, 16, 24, 40, 51, 61, 12, 12, 14, 19, 26, 58, 60, 55,
+    14, 13, 16, 24, 40, 57, 69, 56, 14, 17, 22, 29, 51, 87, 80, 62,
+    18, 22, 37, 56, 68, 109, 103, 77, 24, 35, 55, 64, 81, 104, 113, 92,
+    49, 64, 78, 87, 103, 121, 120, 101, 72, 92, 95, 98, 112, 100, 103, 99], np.float32)
+
+
+def leg_codec(hip):
+    """8x8 block codec step on an 8192x8192 u8 image (a smooth synthetic field plus noise)
+    with the T.81 Annex K luminance table: encode (u8 -> level shift, 8x8 DCT, quantize ->
+    int16) and decode, each one kernel; 3 B per pixel of HBM traffic per direction."""
+    h = w = 8192
+    rng = np.random.default_rng(11)
+    r = np.arange(h, dtype=np.float32)[:, None]
+    c = np.arange(w, dtype=np.float32)[None, :]
+    img = np.clip(np.rint(128 + 80 * np.sin(r / 97.0) * np.cos(c / 131.0) +
+                          rng.normal(0, 3, (h, w)).astype(np.float32)), 0, 255).astype(np.uint8)
+    nb = (h // 8) * (w // 8)
+    codec = capi.Codec8(JPEG_LUMA)
+    d_img, d_co, d_out = capi.DeviceBuffer(h * w), capi.DeviceBuffer(nb * 128), capi.DeviceBuffer(h * w)
+    d_img.upload(img)
+    res = {'image': '%dx%d u8' % (h, w), 'quant': 'T.81 Annex K luminance'}
+    for name, fn in (('encode', lambda: codec.encode_device(d_img.ptr, h, w, w, d_co.ptr)),
+                     ('decode', lambda: codec.decode_device(d_co.ptr, h, w, w, d_out.ptr))):
+        ms = hip.time_ms(codec.stream, fn, 10)
+        gbs = 3.0 * h * w / (ms * 1e-3) / 1e9
+        res[name] = {'ms': round(ms, 4), 'gpx_s': round(h * w / (ms * 1e-3) / 1e9, 1), 'gbs': round(gbs, 1),
+                     'hbm_frac': round(gbs / HBM_PEAK_GBS, 4)}
+    assert codec.synchronize() == 0
+    back = np.empty_like(img)
+    d_out.download(back)
+    co = np.empty((nb, 64), np.int16)
+    d_co.download(co)
+    mse = float(np.mean((back.astype(np.float64) - img) ** 2))
+    res['psnr_db'] = round(10 * np.log10(255.0 ** 2 / mse), 2) if mse else None
+    res['nonzero_coeff_frac'] = round(float(np.count_nonzero(co)) / co.size, 4)
+    codec.close()
+    return res
+
+
 def leg_ragged(hip, lib):
     """10,000 float32 arrays, each axis uniform over {8, ..., 1024}, values uniform[-1, 1)
     times a per-array scale 10^U[-3, 3], seed 7: one persistent launch per direction vs one
@@ -398,8 +440,8 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--legs', default='c2,copy,c1,dense,blocks,ragged',
-                    help='comma list of legs: c2 (always run), copy, c1, dense, blocks, ragged')
+    ap.add_argument('--legs', default='c2,copy,c1,dense,blocks,ragged,codec',
+                    help='comma list of legs: c2 (always run), copy, c1, dense, blocks, ragged, codec')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the transform has no multi-GPU path in this '
@@ -494,6 +536,8 @@ def main():
         line['blocks_8x8'] = leg_blocks(hip, lib)
     if 'ragged' in legs:
         line['ragged'] = leg_ragged(hip, lib)
+    if 'codec' in legs:
+        line['codec_8x8'] = leg_codec(hip)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(inputs)
     print(json.dumps(line))

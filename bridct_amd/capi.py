@@ -54,6 +54,18 @@ SIGNATURES = {
     'bridct_batch_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     'bridct_batch_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_batch_set_spin_limit': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
+    'bridct_codec8_create': (ctypes.c_void_p, [ctypes.c_void_p]),
+    'bridct_codec8_destroy': (None, [ctypes.c_void_p]),
+    'bridct_codec8_encode_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                   ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p]),
+    'bridct_codec8_decode_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                   ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p]),
+    'bridct_codec8_encode': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                            ctypes.c_size_t, ctypes.c_void_p]),
+    'bridct_codec8_decode': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                            ctypes.c_size_t, ctypes.c_void_p]),
+    'bridct_codec8_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
+    'bridct_codec8_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_gpu_malloc': (ctypes.c_void_p, [ctypes.c_size_t]),
     'bridct_gpu_free': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_gpu_upload': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
@@ -220,6 +232,64 @@ class Batch:
     def close(self):
         if self.ptr:
             self.lib.bridct_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Codec8:
+    """Owning wrapper of a bridct_codec8 with quantizer steps `quant` (64 values, row-major
+    u * 8 + v).  encode: u8 image (H, W) -> int16 coefficients (H/8 * W/8, 64); decode: the
+    reverse."""
+
+    def __init__(self, quant):
+        import numpy as np
+        self.lib = load()
+        self.quant = np.ascontiguousarray(quant, dtype=np.float32).reshape(64)
+        self.ptr = self.lib.bridct_codec8_create(ptr(self.quant))
+        if not self.ptr:
+            raise RuntimeError('bridct_codec8_create failed')
+
+    def encode_device(self, d_image, height, width, stride, d_coeffs):
+        return self.lib.bridct_codec8_encode_device(self.ptr, d_image, height, width, stride, d_coeffs)
+
+    def decode_device(self, d_coeffs, height, width, stride, d_image):
+        return self.lib.bridct_codec8_decode_device(self.ptr, d_coeffs, height, width, stride, d_image)
+
+    def encode(self, image):
+        """Host call: image is a 2D uint8 array whose rows may be strided."""
+        import numpy as np
+        assert image.dtype == np.uint8 and image.ndim == 2 and image.strides[1] == 1
+        h, w = image.shape
+        out = np.empty(((h // 8) * (w // 8), 64), np.int16)
+        rc = self.lib.bridct_codec8_encode(self.ptr, ptr(image), h, w, image.strides[0], ptr(out))
+        if rc:
+            raise RuntimeError('bridct_codec8_encode: status %d' % rc)
+        return out
+
+    def decode(self, coeffs, height, width):
+        import numpy as np
+        coeffs = np.ascontiguousarray(coeffs, np.int16)
+        out = np.empty((height, width), np.uint8)
+        rc = self.lib.bridct_codec8_decode(self.ptr, ptr(coeffs), height, width, width, ptr(out))
+        if rc:
+            raise RuntimeError('bridct_codec8_decode: status %d' % rc)
+        return out
+
+    def synchronize(self):
+        return self.lib.bridct_codec8_synchronize(self.ptr)
+
+    @property
+    def stream(self):
+        return self.lib.bridct_codec8_stream(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.lib.bridct_codec8_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

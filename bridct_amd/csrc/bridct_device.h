@@ -138,6 +138,15 @@ int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, i
                         int prec, const float *d_qh, const float *d_qht, const float *d_qw, float scale,
                         void *stream);
 
+/* 8x8 block codec step over a height x width u8 raster (multiples of 8; row stride
+ * `stride` bytes, a multiple of 8; 8-byte aligned) and its block-contiguous int16
+ * coefficients (64 per block, 16-byte aligned).  decode = 0: d_src is the raster, d_dst the
+ * coefficients; decode = 1: the reverse.  d_factors: 64 encode factors
+ * alpha_u alpha_v / Q_uv, then 64 decode factors Q_uv alpha_u alpha_v (alpha: the
+ * length-8 core normalization). */
+int bdx_codec8(int decode, const void *d_src, void *d_dst, size_t height, size_t width, size_t stride,
+               const float *d_factors, void *stream);
+
 /* One kernel of a FORWARD or INVERSE transform: pass 0 = row pass (d_in -> d_out; the
  * whole transform for shapes with a whole-array kernel), pass 1 = column pass in place
  * (d_in == d_out expected; nothing for whole-array shapes). */

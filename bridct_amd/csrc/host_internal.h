@@ -7,6 +7,10 @@
  * evaluated in FP64 and rounded once. */
 __attribute__((visibility("hidden"))) void bridct_build_tables(float *t);
 
+/* Per-axis normalization of the length-n core, output index k, in FP64 (the values the
+ * device tables hold rounded to FP32). */
+__attribute__((visibility("hidden"))) double bridct_axis_alpha(int n, int k);
+
 /* Route and whole-array packing of shape (h, w) from the schedule table; 0 if found. */
 __attribute__((visibility("hidden"))) int bridct_schedule_lookup(int h, int w, int *route, int *pack);
 

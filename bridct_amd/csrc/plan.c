@@ -76,7 +76,7 @@ static const double *sj_factor(int n)
  *    so the factor is s_{4n,k} d_n[k] sqrt(...).
  *  - n >= 128: the core returns C_II[k] / sigma_k with sigma_k = cos(pi k' / (2n)),
  *    k' = min(k, n - k), halved except at k in {0, n/4, n/2, 3n/4}. */
-static double axis_alpha(int n, int k)
+double bridct_axis_alpha(int n, int k)
 {
     const double q = sqrt((k ? 2.0 : 1.0) / n);
     if (n <= 64)
@@ -99,7 +99,7 @@ void bridct_build_tables(float *t)
         for (int k = 0; k < n; k++)
             b[2 * n + k] = k <= n / 2 ? (float)tan(PI * k / (2.0 * n)) : 0.0f;
         for (int k = 0; k < n; k++) {
-            const double a = axis_alpha(n, k);
+            const double a = bridct_axis_alpha(n, k);
             b[3 * n + k] = (float)a;
             b[4 * n + k] = (float)(a * a);
         }

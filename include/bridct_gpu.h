@@ -114,6 +114,41 @@ BRIDCT_API void *bridct_batch_stream(const bridct_batch *batch);
  * small bound makes that path testable.  Takes effect from the next call. */
 BRIDCT_API int bridct_batch_set_spin_limit(bridct_batch *batch, unsigned limit);
 
+/* ---- 8x8 block codec step ----
+ * No counterpart in the reference: the stage a block codec runs right after the forward
+ * transform (encode) and right before the inverse (decode), fused with the 8x8 transform.
+ *   encode: a u8 raster image, height x width pixels (both multiples of 8) with rows
+ *           `stride` bytes apart -> for every 8x8 block, in raster order of the block
+ *           grid, 64 int16 coefficients q[u*8+v] = round-half-even(Y[u][v] / Q[u*8+v]),
+ *           saturated to int16, where Y is the orthonormal 2D DCT-II of the block minus
+ *           128 (the plan FORWARD transform of an 8x8 array);
+ *   decode: the coefficients -> X = orthonormal 2D DCT-III of (q * Q) (the plan INVERSE
+ *           transform) -> pixels round-half-even(X + 128) saturated to 0..255.
+ * Q (64 quantizer steps, row-major u*8+v) is fixed at creation; every step must be finite
+ * and positive.  Arithmetic is FP32, with the transform normalization and 1/Q (or Q)
+ * folded into one factor per coefficient, so a value within FP32 rounding of a tie may
+ * round to either neighbour.  A codec object, like a plan, must not be used from two
+ * threads at once. */
+typedef struct bridct_codec8 bridct_codec8;
+/* NULL when a step is not finite and positive, no GPU is usable, or allocation fails. */
+BRIDCT_API bridct_codec8 *bridct_codec8_create(const float quant[64]);
+BRIDCT_API void bridct_codec8_destroy(bridct_codec8 *codec);   /* NULL is a no-op */
+/* Device buffers, asynchronous on the codec's stream: d_image 8-byte aligned with `stride`
+ * a multiple of 8 and at least width; d_coeffs 16-byte aligned, (height/8)(width/8)*64
+ * int16.  BRIDCT_INVALID_ARGUMENT for sizes that are zero or not multiples of 8,
+ * BRIDCT_BAD_ALIGNMENT for misaligned pointers or stride. */
+BRIDCT_API int bridct_codec8_encode_device(bridct_codec8 *codec, const unsigned char *d_image, size_t height,
+                                           size_t width, size_t stride, short *d_coeffs);
+BRIDCT_API int bridct_codec8_decode_device(bridct_codec8 *codec, const short *d_coeffs, size_t height,
+                                           size_t width, size_t stride, unsigned char *d_image);
+/* Host buffers, synchronous (copied through device buffers kept by the codec). */
+BRIDCT_API int bridct_codec8_encode(bridct_codec8 *codec, const unsigned char *image, size_t height,
+                                    size_t width, size_t stride, short *coeffs);
+BRIDCT_API int bridct_codec8_decode(bridct_codec8 *codec, const short *coeffs, size_t height, size_t width,
+                                    size_t stride, unsigned char *image);
+BRIDCT_API int bridct_codec8_synchronize(bridct_codec8 *codec);
+BRIDCT_API void *bridct_codec8_stream(const bridct_codec8 *codec);
+
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);
 BRIDCT_API int bridct_gpu_free(void *d_ptr);

@@ -144,3 +144,84 @@ int oracle_dct2d(int h, int w, int mode, const double *x, double *y)
     free(t);
     return 0;
 }
+
+/* ---- 8x8 block codec step ------------------------------------------------------------
+ * Not a reference algorithm: the caller-side stage of a block codec around the 8x8 plan
+ * transform (docs/API.md:9-12 for the transform).  Quantization as in baseline JPEG
+ * (ITU-T T.81, A.3.4): coefficient Y / Q rounded to nearest, ties to even here, saturated
+ * to int16; decode multiplies back, inverts, adds 128, rounds and saturates to 0..255. */
+static double round_half_even(double v)
+{
+    double f = floor(v), d = v - f;
+    if (d > 0.5)
+        return f + 1.0;
+    if (d < 0.5)
+        return f;
+    return fmod(f, 2.0) == 0.0 ? f : f + 1.0;
+}
+
+static double clampd(double v, double lo, double hi)
+{
+    return v < lo ? lo : v > hi ? hi : v;
+}
+
+int oracle_codec8_encode(const unsigned char *img, size_t height, size_t width, size_t stride,
+                         const double *quant, double *quotient, short *coeffs)
+{
+    if (!img || !quant || !coeffs || height % 8 || width % 8 || stride < width)
+        return -1;
+    double q8[64], x[64], y[64];
+    oracle_qmatrix(8, q8);
+    const size_t bw = width / 8, nb = (height / 8) * bw;
+    for (size_t b = 0; b < nb; b++) {
+        const size_t by = b / bw, bx = b % bw;
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++)
+                x[r * 8 + c] = (double)img[(by * 8 + r) * stride + bx * 8 + c] - 128.0;
+        /* Y = Q X Q^T */
+        for (int u = 0; u < 8; u++)
+            for (int v = 0; v < 8; v++) {
+                double acc = 0.0;
+                for (int r = 0; r < 8; r++)
+                    for (int c = 0; c < 8; c++)
+                        acc += q8[u * 8 + r] * x[r * 8 + c] * q8[v * 8 + c];
+                y[u * 8 + v] = acc;
+            }
+        for (int i = 0; i < 64; i++) {
+            const double t = y[i] / quant[i];
+            if (quotient)
+                quotient[b * 64 + i] = t;
+            coeffs[b * 64 + i] = (short)clampd(round_half_even(t), -32768.0, 32767.0);
+        }
+    }
+    return 0;
+}
+
+int oracle_codec8_decode(const short *coeffs, size_t height, size_t width, size_t stride,
+                         const double *quant, double *value, unsigned char *img)
+{
+    if (!img || !quant || !coeffs || height % 8 || width % 8 || stride < width)
+        return -1;
+    double q8[64], y[64];
+    oracle_qmatrix(8, q8);
+    const size_t bw = width / 8, nb = (height / 8) * bw;
+    for (size_t b = 0; b < nb; b++) {
+        const size_t by = b / bw, bx = b % bw;
+        for (int i = 0; i < 64; i++)
+            y[i] = (double)coeffs[b * 64 + i] * quant[i];
+        /* X = Q^T Y Q */
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                double acc = 0.0;
+                for (int u = 0; u < 8; u++)
+                    for (int v = 0; v < 8; v++)
+                        acc += q8[u * 8 + r] * y[u * 8 + v] * q8[v * 8 + c];
+                const double t = acc + 128.0;
+                const size_t row = by * 8 + r, col = bx * 8 + c;
+                if (value)
+                    value[row * width + col] = t;
+                img[row * stride + col] = (unsigned char)clampd(round_half_even(t), 0.0, 255.0);
+            }
+    }
+    return 0;
+}

@@ -142,3 +142,63 @@ def families(h, w, rng):
             ('neighbors', (1 + alt * 2 ** -23).astype(np.float32)),
             ('tiny', (noise * 2. ** -70).astype(np.float32)),
             ('large', (noise * 2. ** 50).astype(np.float32))]
+
+
+# ---- 8x8 block codec step ----------------------------------------------------------------
+
+# Luminance quantization table of ITU-T T.81 Annex K (Table K.1), row-major u * 8 + v.
+JPEG_LUMA = np.array([
+    16, 11, 10, 16, 24, 40, 51, 61, 12, 12, 14, 19, 26, 58, 60, 55,
+    14, 13, 16, 24, 40, 57, 69, 56, 14, 17, 22, 29, 51, 87, 80, 62,
+    18, 22, 37, 56, 68, 109, 103, 77, 24, 35, 55, 64, 81, 104, 113, 92,
+    49, 64, 78, 87, 103, 121, 120, 101, 72, 92, 95, 98, 112, 100, 103, 99], np.float32)
+
+
+def _codec_lib():
+    lib = oracle()
+    if not getattr(lib, '_codec_typed', False):
+        lib.oracle_codec8_encode.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.oracle_codec8_decode.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib._codec_typed = True
+    return lib
+
+
+def oracle_encode8(img, quant):
+    """FP64 codec encode of a uint8 (H, W) image (rows may be strided): (coefficients
+    (nblocks, 64) int16, unrounded quotients Y / Q (nblocks, 64))."""
+    h, w = img.shape
+    assert img.dtype == np.uint8 and img.strides[1] == 1
+    q = np.ascontiguousarray(quant, np.float64).reshape(64)
+    nb = (h // 8) * (w // 8)
+    coeffs = np.empty((nb, 64), np.int16)
+    quot = np.empty((nb, 64), np.float64)
+    assert _codec_lib().oracle_codec8_encode(img.ctypes.data, h, w, img.strides[0], q.ctypes.data,
+                                             quot.ctypes.data, coeffs.ctypes.data) == 0
+    return coeffs, quot
+
+
+def oracle_decode8(coeffs, h, w, quant):
+    """FP64 codec decode: (uint8 image (H, W), unrounded X + 128 (H, W))."""
+    c = np.ascontiguousarray(coeffs, np.int16)
+    q = np.ascontiguousarray(quant, np.float64).reshape(64)
+    img = np.empty((h, w), np.uint8)
+    val = np.empty((h, w), np.float64)
+    assert _codec_lib().oracle_codec8_decode(c.ctypes.data, h, w, w, q.ctypes.data, val.ctypes.data,
+                                             img.ctypes.data) == 0
+    return img, val
+
+
+def rounding_agrees(got, want, unrounded, band):
+    """Integer results from FP32 arithmetic against the FP64 oracle: equal wherever the
+    oracle's unrounded value lies more than `band` from a rounding boundary (k + 1/2);
+    within the band the FP32 value may round to the neighbour.  Returns the number of
+    positions inside the band (all of which differ by at most 1)."""
+    got = np.asarray(got, np.int64)
+    want = np.asarray(want, np.int64)
+    near = np.abs(np.abs(unrounded - np.floor(unrounded)) - 0.5) <= band
+    bad = (got != want) & ~near
+    assert not bad.any(), 'mismatch away from a rounding boundary at %s' % (np.argwhere(bad)[:5].tolist(),)
+    assert (np.abs(got - want) <= 1).all()
+    return int(near.sum())
