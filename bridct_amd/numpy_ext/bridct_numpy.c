@@ -20,6 +20,13 @@
  * forward / inverse / roundtrip(arrays, *, out=None) take a list or tuple of float32
  * (h_i, w_i) arrays, validated like Plan inputs, and return a list (or fill `out`, a list
  * of writable arrays of the same shapes, each disjoint from its input).
+ *
+ * Codec8(quant) has no counterpart in the reference either: the 8x8 block codec step
+ * (bridct_codec8_*).  quant: float32 ndarray of 64 finite positive steps (shape (64,) or
+ * (8, 8), row-major u * 8 + v).  encode(image, *, out=None): uint8 (H, W) image, H and W
+ * multiples of 8, rows may be strided -> int16 (H/8, W/8, 8, 8) quantized coefficients;
+ * decode(coeffs, *, out=None): the reverse (out may have strided rows).  Same exception
+ * split, busy flag and GIL rule as Plan.
  */
 #define PY_SSIZE_T_CLEAN
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
@@ -29,6 +36,7 @@
 #include "bridct.h"
 #include "bridct_gpu.h"
 
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -597,6 +605,278 @@ static PyTypeObject RaggedPlanType = {
     .tp_init = (initproc)RaggedPlan_init,
     .tp_new = PyType_GenericNew};
 
+/* ---- Codec8 ---------------------------------------------------------------------------- */
+
+typedef struct {
+    PyObject_HEAD
+    bridct_codec8 *codec;
+    float quant[64];
+    int initialized, busy;
+} Codec8;
+
+static int Codec8_init(Codec8 *self, PyObject *args, PyObject *kwargs)
+{
+    static char *keywords[] = {"quant", NULL};
+    PyObject *q_obj;
+    if (self->initialized) {
+        PyErr_SetString(PyExc_RuntimeError, "A Codec8 cannot be reinitialized.");
+        return -1;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is already initializing or executing.");
+        return -1;
+    }
+    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "O:Codec8", keywords, &q_obj))
+        return -1;
+    if (!PyArray_CheckExact(q_obj)) {
+        PyErr_SetString(PyExc_TypeError, "Expected the 64 quantizer steps as an exact NumPy ndarray.");
+        return -1;
+    }
+    PyArrayObject *q = (PyArrayObject *)q_obj;
+    if (PyArray_TYPE(q) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(q)) {
+        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32 quantizer steps.");
+        return -1;
+    }
+    if (PyArray_SIZE(q) != 64 || !(PyArray_NDIM(q) == 1 || (PyArray_NDIM(q) == 2 && PyArray_DIM(q, 0) == 8)) ||
+        !PyArray_IS_C_CONTIGUOUS(q) || !PyArray_ISALIGNED(q)) {
+        PyErr_SetString(PyExc_ValueError, "Expected 64 C-contiguous steps, shape (64,) or (8, 8).");
+        return -1;
+    }
+    const float *steps = PyArray_DATA(q);
+    for (int i = 0; i < 64; i++)
+        if (!isfinite(steps[i]) || !(steps[i] > 0.0f)) {
+            PyErr_SetString(PyExc_ValueError, "Every quantizer step must be finite and positive.");
+            return -1;
+        }
+    if (!bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT codec cannot be created.");
+        return -1;
+    }
+    self->busy = 1;
+    bridct_codec8 *codec = bridct_codec8_create(steps);
+    self->busy = 0;
+    if (!codec) {
+        PyErr_NoMemory();
+        return -1;
+    }
+    memcpy(self->quant, steps, sizeof(self->quant));
+    self->codec = codec;
+    self->initialized = 1;
+    return 0;
+}
+
+static void Codec8_dealloc(Codec8 *self)
+{
+    bridct_codec8_destroy(self->codec);
+    Py_TYPE(self)->tp_free((PyObject *)self);
+}
+
+static PyObject *Codec8_close(Codec8 *self, PyObject *unused)
+{
+    (void)unused;
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError,
+                        "This Codec8 is already initializing or executing, possibly in another thread.");
+        return NULL;
+    }
+    bridct_codec8_destroy(self->codec);
+    self->codec = NULL;
+    Py_RETURN_NONE;
+}
+
+/* A uint8 (H, W) raster with unit column stride and rows at least W bytes apart. */
+static PyArrayObject *codec_image(PyObject *object, int writable, npy_intp h, npy_intp w)
+{
+    if (!PyArray_CheckExact(object)) {
+        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+        return NULL;
+    }
+    PyArrayObject *a = (PyArrayObject *)object;
+    if (PyArray_TYPE(a) != NPY_UINT8) {
+        PyErr_SetString(PyExc_TypeError, "Expected a uint8 image.");
+        return NULL;
+    }
+    if (PyArray_NDIM(a) != 2 || PyArray_DIM(a, 0) < 8 || PyArray_DIM(a, 1) < 8 || PyArray_DIM(a, 0) % 8 ||
+        PyArray_DIM(a, 1) % 8 || (h >= 0 && (PyArray_DIM(a, 0) != h || PyArray_DIM(a, 1) != w))) {
+        PyErr_SetString(PyExc_ValueError, "Expected a (height, width) image, both multiples of 8 (and matching).");
+        return NULL;
+    }
+    if (PyArray_STRIDE(a, 1) != 1 || PyArray_STRIDE(a, 0) < PyArray_DIM(a, 1)) {
+        PyErr_SetString(PyExc_ValueError, "Expected contiguous rows (unit column stride, forward row stride).");
+        return NULL;
+    }
+    if (writable && !PyArray_ISWRITEABLE(a)) {
+        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
+        return NULL;
+    }
+    return a;
+}
+
+/* int16 (H/8, W/8, 8, 8) coefficients, C-contiguous. */
+static PyArrayObject *codec_coeffs(PyObject *object, int writable, npy_intp bh, npy_intp bw)
+{
+    if (!PyArray_CheckExact(object)) {
+        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+        return NULL;
+    }
+    PyArrayObject *a = (PyArrayObject *)object;
+    if (PyArray_TYPE(a) != NPY_INT16 || !PyArray_ISNOTSWAPPED(a)) {
+        PyErr_SetString(PyExc_TypeError, "Expected native-endian int16 coefficients.");
+        return NULL;
+    }
+    if (PyArray_NDIM(a) != 4 || PyArray_DIM(a, 0) < 1 || PyArray_DIM(a, 1) < 1 || PyArray_DIM(a, 2) != 8 ||
+        PyArray_DIM(a, 3) != 8 || (bh >= 0 && (PyArray_DIM(a, 0) != bh || PyArray_DIM(a, 1) != bw))) {
+        PyErr_SetString(PyExc_ValueError, "Expected (height/8, width/8, 8, 8) coefficients (and matching).");
+        return NULL;
+    }
+    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
+        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, aligned coefficients.");
+        return NULL;
+    }
+    if (writable && !PyArray_ISWRITEABLE(a)) {
+        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
+        return NULL;
+    }
+    return a;
+}
+
+/* First and one-past-last byte of an array's storage (non-negative strides). */
+static void extent(PyArrayObject *a, uintptr_t *lo, uintptr_t *hi)
+{
+    uintptr_t end = (uintptr_t)PyArray_DATA(a);
+    for (int d = 0; d < PyArray_NDIM(a); d++)
+        end += (uintptr_t)(PyArray_DIM(a, d) - 1) * (uintptr_t)PyArray_STRIDE(a, d);
+    *lo = (uintptr_t)PyArray_DATA(a);
+    *hi = end + (uintptr_t)PyArray_ITEMSIZE(a);
+}
+
+static PyObject *codec_execute(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
+                               int decode)
+{
+    if (!self->codec) {
+        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is closed or uninitialized.");
+        return NULL;
+    }
+    if (self->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is executing in another thread; use one per thread.");
+        return NULL;
+    }
+    self->busy = 1;
+    PyArrayObject *out = NULL;
+    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
+    if (nargs != 1 || nkw > 1) {
+        PyErr_SetString(PyExc_TypeError, "Use method(input, *, out=None).");
+        goto fail;
+    }
+    PyObject *out_obj = Py_None;
+    if (nkw) {
+        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
+            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
+            goto fail;
+        }
+        out_obj = args[1];
+    }
+    PyArrayObject *in = decode ? codec_coeffs(args[0], 0, -1, -1) : codec_image(args[0], 0, -1, -1);
+    if (!in)
+        goto fail;
+    const npy_intp h = decode ? 8 * PyArray_DIM(in, 0) : PyArray_DIM(in, 0);
+    const npy_intp w = decode ? 8 * PyArray_DIM(in, 1) : PyArray_DIM(in, 1);
+    if (out_obj == Py_None) {
+        npy_intp img_dims[2] = {h, w}, co_dims[4] = {h / 8, w / 8, 8, 8};
+        out = (PyArrayObject *)(decode ? PyArray_SimpleNew(2, img_dims, NPY_UINT8)
+                                       : PyArray_SimpleNew(4, co_dims, NPY_INT16));
+        if (!out)
+            goto fail;
+    } else {
+        PyArrayObject *given = decode ? codec_image(out_obj, 1, h, w) : codec_coeffs(out_obj, 1, h / 8, w / 8);
+        if (!given)
+            goto fail;
+        uintptr_t a0, a1, b0, b1;
+        extent(in, &a0, &a1);
+        extent(given, &b0, &b1);
+        if (a0 < b1 && b0 < a1) {
+            PyErr_SetString(PyExc_ValueError, "Input and output must not overlap.");
+            goto fail;
+        }
+        out = given;
+        Py_INCREF(out);
+    }
+    PyArrayObject *image = decode ? out : in;
+    const size_t stride = (size_t)PyArray_STRIDE(image, 0);
+    int status;
+    Py_BEGIN_ALLOW_THREADS
+    if (decode)
+        status = bridct_codec8_decode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
+                                      PyArray_DATA(out));
+    else
+        status = bridct_codec8_encode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
+                                      PyArray_DATA(out));
+    Py_END_ALLOW_THREADS
+    if (status) {
+        PyErr_Format(PyExc_RuntimeError, "BRiDCT codec failed with status %d.", status);
+        goto fail;
+    }
+    self->busy = 0;
+    return (PyObject *)out;
+fail:
+    self->busy = 0;
+    Py_XDECREF(out);
+    return NULL;
+}
+
+static PyObject *Codec8_encode(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return codec_execute(self, args, nargs, kw, 0);
+}
+
+static PyObject *Codec8_decode(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+{
+    return codec_execute(self, args, nargs, kw, 1);
+}
+
+static PyObject *Codec8_quant(Codec8 *self, void *unused)
+{
+    (void)unused;
+    if (!self->initialized)
+        Py_RETURN_NONE;
+    npy_intp dims[2] = {8, 8};
+    PyObject *q = PyArray_SimpleNew(2, dims, NPY_FLOAT32);
+    if (q)
+        memcpy(PyArray_DATA((PyArrayObject *)q), self->quant, sizeof(self->quant));
+    return q;
+}
+
+static PyObject *Codec8_closed(Codec8 *self, void *unused)
+{
+    (void)unused;
+    return PyBool_FromLong(self->codec == NULL);
+}
+
+static PyMethodDef Codec8_methods[] = {
+    {"encode", AS_METHOD(Codec8_encode), METH_FASTCALL | METH_KEYWORDS,
+     "encode(image, *, out=None): uint8 (H, W) -> int16 (H/8, W/8, 8, 8) quantized 8x8 DCT coefficients."},
+    {"decode", AS_METHOD(Codec8_decode), METH_FASTCALL | METH_KEYWORDS,
+     "decode(coeffs, *, out=None): int16 (H/8, W/8, 8, 8) -> uint8 (H, W) image."},
+    {"close", AS_METHOD(Codec8_close), METH_NOARGS, "Release the GPU codec and its buffers."},
+    {NULL, NULL, 0, NULL}};
+
+static PyGetSetDef Codec8_getset[] = {
+    {"quant", (getter)Codec8_quant, NULL, "The quantizer steps, (8, 8) float32.", NULL},
+    {"closed", (getter)Codec8_closed, NULL, "Whether the codec has been closed.", NULL},
+    {NULL, NULL, NULL, NULL, NULL}};
+
+static PyTypeObject Codec8Type = {
+    PyVarObject_HEAD_INIT(NULL, 0)
+    .tp_name = "bridct_numpy.Codec8",
+    .tp_basicsize = sizeof(Codec8),
+    .tp_dealloc = (destructor)Codec8_dealloc,
+    .tp_flags = Py_TPFLAGS_DEFAULT,
+    .tp_doc = "8x8 block codec step on the GPU: level shift, 8x8 DCT and quantization, and back.",
+    .tp_methods = Codec8_methods,
+    .tp_getset = Codec8_getset,
+    .tp_init = (initproc)Codec8_init,
+    .tp_new = PyType_GenericNew};
+
 static int module_exec(PyObject *m)
 {
     if (PyInterpreterState_Get() != PyInterpreterState_Main()) {
@@ -617,6 +897,13 @@ static int module_exec(PyObject *m)
     Py_INCREF(&RaggedPlanType);
     if (PyModule_AddObject(m, "RaggedPlan", (PyObject *)&RaggedPlanType) < 0) {
         Py_DECREF(&RaggedPlanType);
+        return -1;
+    }
+    if (PyType_Ready(&Codec8Type) < 0)
+        return -1;
+    Py_INCREF(&Codec8Type);
+    if (PyModule_AddObject(m, "Codec8", (PyObject *)&Codec8Type) < 0) {
+        Py_DECREF(&Codec8Type);
         return -1;
     }
     if (PyModule_AddStringConstant(m, "core_version", BRIDCT_VERSION) < 0)

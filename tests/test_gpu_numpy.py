@@ -3,7 +3,8 @@ NumPy interface (python/test_api.py): rejected inputs and their exception types,
 ownership and `out=` guards, closed / uninitialized / reinitialized plans, numerical error
 against the FP64 DCT (scipy) on the reference's seeded families, roundtrip ==
 inverse(forward) bitwise, independent plans in threads, and the busy rejection of a plan
-shared by two threads.  Plus the RaggedPlan list entry, which the reference does not have."""
+shared by two threads.  Plus the RaggedPlan list entry and the Codec8 block codec step, which
+the reference does not have."""
 import concurrent.futures
 import gc
 import threading
@@ -192,3 +193,59 @@ def test_ragged_plan(npmod):
     rejected(lambda: npmod.RaggedPlan([]), ValueError)
     rejected(lambda: npmod.RaggedPlan([(8, 8), (12, 8)]), ValueError)
     rejected(lambda: npmod.RaggedPlan([8, 8]), TypeError)
+
+
+def test_codec8(npmod):
+    """Codec8: the 8x8 block codec step of the C ABI (bridct_codec8_*) from NumPy: results
+    equal the C entry points bitwise, `out=` contract, strided rows, rejected inputs."""
+    import support as S
+    from bridct_amd import capi
+    rng = np.random.default_rng(12)
+    img = rng.integers(0, 256, (64, 96), dtype=np.uint8)
+    codec = npmod.Codec8(S.JPEG_LUMA.reshape(8, 8))
+    ref = capi.Codec8(S.JPEG_LUMA)
+    try:
+        assert not codec.closed and np.array_equal(codec.quant, S.JPEG_LUMA.reshape(8, 8))
+        co = codec.encode(img)
+        assert co.shape == (8, 12, 8, 8) and co.dtype == np.int16 and co.flags.owndata
+        assert np.array_equal(co.reshape(-1, 64), ref.encode(img))
+        pix = codec.decode(co)
+        assert pix.shape == (64, 96) and pix.dtype == np.uint8
+        assert np.array_equal(pix, ref.decode(co.reshape(-1, 64), 64, 96))
+        # strided rows in and out
+        big = np.zeros((80, 128), np.uint8)
+        big[8:72, 16:112] = img
+        assert np.array_equal(codec.encode(big[8:72, 16:112]), co)
+        canvas = np.full((64, 120), 9, np.uint8)
+        assert codec.decode(co, out=canvas[:, 8:104]) is not None
+        assert np.array_equal(canvas[:, 8:104], pix) and np.all(canvas[:, :8] == 9) and np.all(canvas[:, 104:] == 9)
+        out = np.empty_like(co)
+        assert codec.encode(img, out=out) is out and np.array_equal(out, co)
+        # rejected inputs
+        for bad in (img.astype(np.int16), img.tolist()):
+            rejected(lambda bad=bad: codec.encode(bad), TypeError)
+        for bad in (img[:60], img[:, :90], img[:, ::2], img.reshape(1, 64, 96), np.zeros((0, 8), np.uint8)):
+            rejected(lambda bad=bad: codec.encode(bad), ValueError)
+        rejected(lambda: codec.decode(co.astype(np.int32)), TypeError)
+        rejected(lambda: codec.decode(co[:, :, :4]), ValueError)
+        rejected(lambda: codec.decode(co[:, ::2]), ValueError)
+        rejected(lambda: codec.encode(img, out=np.empty((8, 12, 8, 8), np.int32)), TypeError)
+        rejected(lambda: codec.encode(img, out=np.empty((8, 11, 8, 8), np.int16)), ValueError)
+        storage = np.zeros(64 * 96 * 3, np.uint8)
+        view_img = storage[:64 * 96].reshape(64, 96)
+        view_co = storage[64:64 + 64 * 96 * 2].view(np.int16).reshape(8, 12, 8, 8)
+        rejected(lambda: codec.encode(view_img, out=view_co), ValueError)
+        ro = np.empty_like(co)
+        ro.flags.writeable = False
+        rejected(lambda: codec.encode(img, out=ro), ValueError)
+        rejected(lambda: codec.encode(img, scale=2), TypeError)
+        rejected(lambda: codec.__init__(S.JPEG_LUMA), RuntimeError)
+    finally:
+        codec.close()
+        ref.close()
+    codec.close()
+    assert codec.closed
+    rejected(lambda: codec.encode(img), RuntimeError)
+    rejected(lambda: npmod.Codec8(np.zeros(64, np.float32)), ValueError)
+    rejected(lambda: npmod.Codec8(np.ones(64)), TypeError)
+    rejected(lambda: npmod.Codec8(np.ones(60, np.float32)), ValueError)
