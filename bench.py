@@ -390,8 +390,9 @@ def leg_codec(hip):
 
 def leg_ragged(hip, lib):
     """10,000 float32 arrays, each axis uniform over {8, ..., 1024}, values uniform[-1, 1)
-    times a per-array scale 10^U[-3, 3], seed 7: one persistent launch per direction vs one
-    plan launch per array (the same FP32 butterfly kernels)."""
+    times a per-array scale 10^U[-3, 3], seed 7: the batch object (one persistent launch per
+    direction, and the grouped per-shape kernels; `fwd` / `inv` report the automatic choice)
+    vs one plan launch per array (the same FP32 butterfly kernels)."""
     rng = np.random.default_rng(7)
     lengths = [8, 16, 32, 64, 128, 256, 512, 1024]
     n = 10000
@@ -402,20 +403,28 @@ def leg_ragged(hip, lib):
         x[o:o + h * w] = rng.uniform(-1, 1, h * w) * 10 ** rng.uniform(-3, 3)
     da, db = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
     da.upload(x)
-    res = {'arrays': n, 'pixels': int(x.size)}
-    for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
-        ms = hip.time_ms(batch.stream, lambda: batch.apply_device(mode, da.ptr, db.ptr), 3)
-        res[name] = {'ms': round(ms, 3), 'gpx_s': round(x.size / (ms * 1e-3) / 1e9, 1),
-                     'hbm_frac': round(8.0 * x.size / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    gms = hip.time_ms(batch.stream, lambda: batch.graph_apply(capi.FORWARD_THEN_INVERSE, da.ptr, db.ptr), 3)
-    res['graph_fwd_then_inv'] = {'ms': round(gms, 3), 'gpx_s': round(x.size / (gms * 1e-3) / 1e9, 1)}
-    assert batch.synchronize() == 0
-    y = np.empty_like(x)
-    db.download(y)
+    res = {'arrays': n, 'pixels': int(x.size), 'strategy_auto': {1: 'persistent', 2: 'grouped'}[batch.strategy]}
     worst = 0.0
-    for o, (h, w) in zip(batch.offsets, shapes):
-        a, b = x[o:o + h * w], y[o:o + h * w]
-        worst = max(worst, float(np.abs(b - a).max() / np.abs(a).max()))
+    for sname, strategy in (('persistent', capi.BATCH_PERSISTENT), ('grouped', capi.BATCH_GROUPED)):
+        assert batch.set_strategy(strategy) == 0
+        r = {}
+        for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
+            ms = hip.time_ms(batch.stream, lambda: batch.apply_device(mode, da.ptr, db.ptr), 3)
+            r[name] = {'ms': round(ms, 3), 'gpx_s': round(x.size / (ms * 1e-3) / 1e9, 1),
+                       'hbm_frac': round(8.0 * x.size / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        gms = hip.time_ms(batch.stream, lambda: batch.graph_apply(capi.FORWARD_THEN_INVERSE, da.ptr, db.ptr), 3)
+        r['graph_fwd_then_inv'] = {'ms': round(gms, 3), 'gpx_s': round(x.size / (gms * 1e-3) / 1e9, 1)}
+        assert batch.synchronize() == 0
+        y = np.empty_like(x)
+        db.download(y)
+        for o, (h, w) in zip(batch.offsets, shapes):
+            a, b = x[o:o + h * w], y[o:o + h * w]
+            worst = max(worst, float(np.abs(b - a).max() / np.abs(a).max()))
+        res[sname] = r
+    assert batch.set_strategy(capi.BATCH_AUTO) == 0
+    # the automatic choice is the headline of this leg
+    best = res['grouped' if batch.strategy == capi.BATCH_GROUPED else 'persistent']
+    res['fwd'], res['inv'] = best['fwd'], best['inv']
     res['roundtrip_max_rel_err'] = worst
     plans = {s: capi.Plan(*s) for s in sorted(set(shapes))}
     for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
@@ -426,8 +435,10 @@ def leg_ragged(hip, lib):
         hip.sync()
         t = time.perf_counter() - t0
         res['per_array_launches_' + name] = {'s': round(t, 4), 'gpx_s': round(x.size / t / 1e9, 2)}
-    res['batched_vs_plain'] = round((res['fwd']['gpx_s'] + res['inv']['gpx_s']) /
-                                    (res['per_array_launches_fwd']['gpx_s'] + res['per_array_launches_inv']['gpx_s']), 2)
+    plain = res['per_array_launches_fwd']['gpx_s'] + res['per_array_launches_inv']['gpx_s']
+    res['batched_vs_plain'] = round((res['fwd']['gpx_s'] + res['inv']['gpx_s']) / plain, 2)
+    res['persistent_vs_plain'] = round((res['persistent']['fwd']['gpx_s'] + res['persistent']['inv']['gpx_s']) /
+                                       plain, 2)
     for p in plans.values():
         p.close()
     batch.close()

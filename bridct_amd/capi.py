@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, 'libbridct_amd.so')
 
 FORWARD, INVERSE, ROUNDTRIP, FORWARD_THEN_INVERSE = 0, 1, 2, 3
 ROUTE_BUTTERFLY, ROUTE_MFMA_F16, ROUTE_MFMA_BF16 = 0, 1, 2
+BATCH_AUTO, BATCH_PERSISTENT, BATCH_GROUPED = 0, 1, 2
 OK, INVALID_ARGUMENT, WORKSPACE_TOO_SMALL, BAD_ALIGNMENT, OVERLAPPING_BUFFERS, DEVICE_ERROR = (
     0, -1, -2, -3, -4, -5)
 
@@ -54,6 +55,8 @@ SIGNATURES = {
     'bridct_batch_apply': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     'bridct_batch_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_batch_set_spin_limit': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
+    'bridct_batch_set_strategy': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    'bridct_batch_get_strategy': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_codec8_create': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_codec8_destroy': (None, [ctypes.c_void_p]),
     'bridct_codec8_encode_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
@@ -220,6 +223,13 @@ class Batch:
 
     def set_spin_limit(self, limit):
         return self.lib.bridct_batch_set_spin_limit(self.ptr, limit)
+
+    def set_strategy(self, strategy):
+        return self.lib.bridct_batch_set_strategy(self.ptr, strategy)
+
+    @property
+    def strategy(self):
+        return self.lib.bridct_batch_get_strategy(self.ptr)
 
     @property
     def stream(self):

@@ -18,8 +18,18 @@
  * item only waits for items earlier in the queue, which were taken by workgroups that are
  * running, so the schedule cannot deadlock.
  *
- * The object owns its stream, device tables, item lists and counters, and one captured
- * graph (hipGraph) for replays of the same (mode, input, output); it has no global state.
+ * A second strategy, GROUPED, runs the arrays of each distinct shape with the plan path's
+ * own per-shape kernels in gather form (the arrays of a shape are not contiguous in the
+ * batch buffer, so the kernels read an offset table): one or two launches per shape,
+ * largest shapes first, all on the batch stream and capturable as one graph.  Each launch
+ * is specialized for its shape (registers, LDS, team size) and needs no cross-workgroup
+ * dependency waits, which makes it faster when every shape has enough pixels to fill the
+ * GPU; the automatic choice takes it when the average shape has at least
+ * GROUPED_MIN_PIXELS pixels.  Batches of tiny arrays always use the tiny kernel.
+ *
+ * The object owns its stream, device tables, item lists and counters, the grouped offset
+ * table, and one captured graph (hipGraph) for replays of the same (mode, input, output);
+ * it has no global state.
  */
 #include "bridct_gpu.h"
 
@@ -34,6 +44,17 @@
  * workgroup, about a second: a legitimate wait is one array's row pass. */
 #define SPIN_LIMIT (1u << 21)
 
+/* Automatic strategy: GROUPED when the batch averages at least this many pixels per
+ * distinct shape (each launch then has several microseconds of work, so launch gaps stay
+ * small against it). */
+#define GROUPED_MIN_PIXELS ((size_t)1 << 21)
+
+/* One distinct shape of a batch: its arrays' offsets are d_goffs[first, first + count). */
+typedef struct {
+    int h, w, pack;
+    size_t count, first;
+} group;
+
 typedef struct {
     bdx_item *items;
     unsigned n, cap;
@@ -46,6 +67,11 @@ struct bridct_batch {
     unsigned nruns, ntwo;    /* runs, two-pass arrays */
     unsigned blocks;
     int tiny;                /* all arrays <= BDX_TINY_MAX on both axes: tiny kernel */
+    int strategy;            /* BRIDCT_BATCH_PERSISTENT or BRIDCT_BATCH_GROUPED */
+    int auto_strategy;       /* what BRIDCT_BATCH_AUTO resolves to */
+    group *groups;           /* distinct shapes, largest total first */
+    unsigned ngroups;
+    size_t *d_goffs;         /* element offsets of the arrays, grouped by shape */
     void *stream;
     float *d_tables;
     bdx_run *d_runs;
@@ -172,14 +198,71 @@ void bridct_batch_destroy(bridct_batch *b)
         bdx_stream_sync(b->stream);
     if (b->graph)
         bdx_graph_destroy(b->graph);
-    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out};
+    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out, b->d_goffs};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             bdx_free(bufs[i]);
     if (b->stream)
         bdx_stream_destroy(b->stream);
     free(b->offsets);
+    free(b->groups);
     free(b);
+}
+
+typedef struct {
+    long key;     /* h * 2048 + w */
+    size_t idx;   /* array index */
+} shape_key;
+
+static int by_shape(const void *pa, const void *pb)
+{
+    const shape_key *a = pa, *b = pb;
+    return a->key < b->key ? -1 : a->key > b->key ? 1 : a->idx < b->idx ? -1 : a->idx > b->idx;
+}
+
+static int by_pixels_desc(const void *pa, const void *pb)
+{
+    const group *a = pa, *b = pb;
+    const size_t na = (size_t)a->h * a->w * a->count, nb = (size_t)b->h * b->w * b->count;
+    return na > nb ? -1 : na < nb ? 1 : 0;
+}
+
+/* Groups the arrays by shape; host offsets into `goffs` (count entries). */
+static int build_groups(bridct_batch *b, const int *heights, const int *widths, size_t *goffs)
+{
+    shape_key *idx = malloc(b->count * sizeof(shape_key));
+    if (!idx)
+        return -1;
+    for (size_t i = 0; i < b->count; i++) {
+        idx[i].key = (long)heights[i] * 2048 + widths[i];
+        idx[i].idx = i;
+    }
+    qsort(idx, b->count, sizeof(shape_key), by_shape);
+    unsigned n = 0;
+    for (size_t i = 0; i < b->count; i++)
+        if (i == 0 || idx[i].key != idx[i - 1].key)
+            n++;
+    b->groups = calloc(n, sizeof(group));
+    if (!b->groups) {
+        free(idx);
+        return -1;
+    }
+    for (size_t i = 0; i < b->count; i++) {
+        const size_t a = idx[i].idx;
+        goffs[i] = b->offsets[a];
+        if (i == 0 || idx[i].key != idx[i - 1].key) {
+            group *g = &b->groups[b->ngroups++];
+            g->h = heights[a];
+            g->w = widths[a];
+            g->first = i;
+            int route = 0, pack = 0;
+            g->pack = bridct_schedule_lookup(g->h, g->w, &route, &pack) == 0 ? pack : 0;
+        }
+        b->groups[b->ngroups - 1].count++;
+    }
+    free(idx);
+    qsort(b->groups, b->ngroups, sizeof(group), by_pixels_desc);
+    return 0;
 }
 
 static int upload(void **dst, const void *src, size_t bytes, void *stream)
@@ -299,24 +382,31 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     b->tiny = sc.tiny;
     float *host_tables = malloc(BDX_TABLE_FLOATS * sizeof(float));
     unsigned *zeros = calloc(b->nctr, sizeof(unsigned));   /* counters start at zero */
-    if (!host_tables || !zeros)
+    size_t *goffs = malloc(b->count * sizeof(size_t));
+    if (!host_tables || !zeros || !goffs || build_groups(b, heights, widths, goffs))
         goto fail;
+    b->auto_strategy = !b->tiny && b->elements / b->ngroups >= GROUPED_MIN_PIXELS ? BRIDCT_BATCH_GROUPED
+                                                                                   : BRIDCT_BATCH_PERSISTENT;
+    b->strategy = b->auto_strategy;
     bridct_build_tables(host_tables);
     if (bdx_stream_create(&b->stream) ||
         upload((void **)&b->d_tables, host_tables, BDX_TABLE_FLOATS * sizeof(float), b->stream) ||
         upload((void **)&b->d_runs, sc.runs, sc.nruns * sizeof(bdx_run), b->stream) ||
         upload((void **)&b->d_items[0], sc.items[0], sc.nitems[0] * sizeof(bdx_item), b->stream) ||
         upload((void **)&b->d_items[1], sc.items[1], sc.nitems[1] * sizeof(bdx_item), b->stream) ||
-        upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) || bdx_stream_sync(b->stream))
+        upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) ||
+        upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) || bdx_stream_sync(b->stream))
         goto fail;
     free(host_tables);
     free(zeros);
+    free(goffs);
     bdx_schedule_free(&sc);
     return b;
 
 fail:
     free(host_tables);
     free(zeros);
+    free(goffs);
     bdx_schedule_free(&sc);
     bridct_batch_destroy(b);
     return NULL;
@@ -346,11 +436,28 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
                             mode, b->d_tables, b->blocks, b->spin_limit, b->tiny, b->stream);
 }
 
+/* GROUPED: the plan path's kernels, per distinct shape, in gather form. */
+static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d_out)
+{
+    for (unsigned i = 0; i < b->ngroups; i++) {
+        const group *g = &b->groups[i];
+        if (bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
+                                 b->d_tables, b->stream))
+            return -1;
+    }
+    return 0;
+}
+
+static int launch_any(bridct_batch *b, int mode, const float *d_in, float *d_out)
+{
+    return b->strategy == BRIDCT_BATCH_GROUPED ? launch_grouped(b, mode, d_in, d_out) : launch(b, mode, d_in, d_out);
+}
+
 static int issue(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
     if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
-        return launch(b, BDX_FORWARD, d_in, d_out) || launch(b, BDX_INVERSE, d_out, d_out);
-    return launch(b, mode, d_in, d_out);
+        return launch_any(b, BDX_FORWARD, d_in, d_out) || launch_any(b, BDX_INVERSE, d_out, d_out);
+    return launch_any(b, mode, d_in, d_out);
 }
 
 static int check_device_args(const bridct_batch *b, int mode, const float *d_in, const float *d_out)
@@ -413,6 +520,12 @@ int bridct_batch_synchronize(bridct_batch *b)
     if (bdx_stream_sync(b->stream) ||
         bdx_copy_d2h(&err, b->d_ctr + BDX_CTR_ERROR, sizeof(err), b->stream) || bdx_stream_sync(b->stream))
         return BRIDCT_DEVICE_ERROR;
+    if (err) {   /* reported once: grouped calls do not reset the flag themselves */
+        const unsigned zero = 0;
+        if (bdx_copy_h2d(b->d_ctr + BDX_CTR_ERROR, &zero, sizeof(zero), b->stream))
+            return BRIDCT_DEVICE_ERROR;
+        bdx_stream_sync(b->stream);
+    }
     return err ? BRIDCT_DEVICE_ERROR : BRIDCT_OK;
 }
 
@@ -453,4 +566,24 @@ int bridct_batch_set_spin_limit(bridct_batch *b, unsigned limit)
 void *bridct_batch_stream(const bridct_batch *b)
 {
     return b ? b->stream : NULL;
+}
+
+int bridct_batch_set_strategy(bridct_batch *b, int strategy)
+{
+    if (!b || strategy < BRIDCT_BATCH_AUTO || strategy > BRIDCT_BATCH_GROUPED)
+        return BRIDCT_INVALID_ARGUMENT;
+    const int resolved = strategy == BRIDCT_BATCH_AUTO ? b->auto_strategy : strategy;
+    if (resolved != b->strategy && b->graph) {   /* the captured sequence changes */
+        if (bdx_stream_sync(b->stream))
+            return BRIDCT_DEVICE_ERROR;
+        bdx_graph_destroy(b->graph);
+        b->graph = NULL;
+    }
+    b->strategy = resolved;
+    return BRIDCT_OK;
+}
+
+int bridct_batch_get_strategy(const bridct_batch *b)
+{
+    return b ? b->strategy : BRIDCT_INVALID_ARGUMENT;
 }

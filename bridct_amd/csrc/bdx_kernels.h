@@ -23,31 +23,34 @@ constexpr bool small_route(int h, int w) { return (long)h * (w + 1) <= kSmallLds
 constexpr int rows_per_block(int w) { return kPassLds / w; }
 constexpr int cols_per_block(int h) { return kPassLds / h > 256 ? 256 : kPassLds / h; }
 
-// mode: 0 forward, 1 inverse, 2 fused round trip.  Return 0 or a negative value.
+// mode: 0 forward, 1 inverse, 2 fused round trip.  Return 0 or a negative value.  With
+// `offs`, array i of the batch is at element offs[i] of in and out (gather form) instead of
+// at i * h * w.
 int launch_small_h8(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h16(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h32(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h64(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h128(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h256(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h512(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 int launch_small_h1024(int w, const float *in, float *out, size_t batch, int pack, int mode, const float *tab,
-                      hipStream_t s);
+                      hipStream_t s, const size_t *offs = nullptr);
 
 // Row pass over nrows rows of length w: op 1 = forward*alpha, 2 = forward*alpha^2,
 // 3 = alpha*inverse, 4 = inverse.
-int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s);
+int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
+                const size_t *offs = nullptr, int h = 0);
 // Column pass over `batch` h x w arrays, in place allowed: mode 0 forward*alpha,
 // 1 alpha*inverse, 2 forward*alpha^2 then inverse.
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
-                hipStream_t s);
+                hipStream_t s, const size_t *offs = nullptr);
 
 // Persistent batch kernel, one per mode (bdx_batch.hip).
 int launch_batch_m0(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,

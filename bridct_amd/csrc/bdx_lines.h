@@ -375,17 +375,16 @@ __device__ __forceinline__ void run_lines(float *tile, int nlines, Base base,
         line_op<n, S, OP>(tile + base(L), t, tab);
 }
 
-// Copies `rows` rows of COLS floats (global row stride gstride floats, 16-byte aligned)
-// into an LDS tile of pitch PITCH, and back.  Global side is float4-coalesced.
-template <int COLS, int PITCH, int THREADS>
-__device__ __forceinline__ void tile_load(float *tile, const float *__restrict__ src, size_t gstride,
-                                          int rows)
+// Copies `rows` rows of COLS floats into an LDS tile of pitch PITCH, and back; row r of
+// the global side starts at row(r) (16-byte aligned).  Global side is float4-coalesced.
+template <int COLS, int PITCH, int THREADS, typename RowPtr>
+__device__ __forceinline__ void tile_load_rows(float *tile, RowPtr row, int rows)
 {
     constexpr int per_row = COLS / 4;
     const int nv = rows * per_row;
     for (int i = threadIdx.x; i < nv; i += THREADS) {
         const int r = i / per_row, c = (i - r * per_row) * 4;
-        const float4 v = *reinterpret_cast<const float4 *>(src + r * gstride + c);
+        const float4 v = *reinterpret_cast<const float4 *>(row(r) + c);
         float *d = tile + r * PITCH + c;
         d[0] = v.x;
         d[1] = v.y;
@@ -394,17 +393,31 @@ __device__ __forceinline__ void tile_load(float *tile, const float *__restrict__
     }
 }
 
-template <int COLS, int PITCH, int THREADS>
-__device__ __forceinline__ void tile_store(const float *tile, float *__restrict__ dst, size_t gstride,
-                                           int rows)
+template <int COLS, int PITCH, int THREADS, typename RowPtr>
+__device__ __forceinline__ void tile_store_rows(const float *tile, RowPtr row, int rows)
 {
     constexpr int per_row = COLS / 4;
     const int nv = rows * per_row;
     for (int i = threadIdx.x; i < nv; i += THREADS) {
         const int r = i / per_row, c = (i - r * per_row) * 4;
         const float *d = tile + r * PITCH + c;
-        *reinterpret_cast<float4 *>(dst + r * gstride + c) = make_float4(d[0], d[1], d[2], d[3]);
+        *reinterpret_cast<float4 *>(row(r) + c) = make_float4(d[0], d[1], d[2], d[3]);
     }
+}
+
+// Contiguous rows, global row stride gstride floats.
+template <int COLS, int PITCH, int THREADS>
+__device__ __forceinline__ void tile_load(float *tile, const float *__restrict__ src, size_t gstride,
+                                          int rows)
+{
+    tile_load_rows<COLS, PITCH, THREADS>(tile, [=](int r) { return src + r * gstride; }, rows);
+}
+
+template <int COLS, int PITCH, int THREADS>
+__device__ __forceinline__ void tile_store(const float *tile, float *__restrict__ dst, size_t gstride,
+                                           int rows)
+{
+    tile_store_rows<COLS, PITCH, THREADS>(tile, [=](int r) { return dst + r * gstride; }, rows);
 }
 
 }  // namespace bdx

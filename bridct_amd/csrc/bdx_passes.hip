@@ -4,6 +4,9 @@
 //   k_rows<W, OP>:  row_unit bands; rows of all arrays of a batch are one sequence.
 //   k_cols<H, MODE>: col_unit strips of one array, in place on the output; one wave per
 //                   team member (column teams), one lane per column below 128.
+// The gather forms (G) take array i of the batch at element offs[i] of in and out instead
+// of at i * h * w (the arrays of one shape inside a ragged batch).  A row band never
+// crosses arrays: every two-pass height is a multiple of the band.
 #include "bdx_units.h"
 
 namespace bdx {
@@ -16,16 +19,24 @@ template <int H> constexpr int col_threads()
     return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
 }
 
-template <int W, int OP>
-__global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, const float *__restrict__ tab)
+template <int W, int OP, bool G>
+__global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, const size_t *__restrict__ offs,
+                                                    int h, const float *__restrict__ tab)
 {
     __shared__ float smem[row_unit_lds<W, OP, kThreads>()];
     constexpr int R = row_unit_rows<W, kThreads>();
-    row_unit<W, OP, kThreads>(in, out, (size_t)blockIdx.x * R, R, smem, tab, false);
+    const size_t row0 = (size_t)blockIdx.x * R;
+    if constexpr (G) {
+        const size_t a = row0 / (unsigned)h, base = offs[a] + (row0 - a * h) * W;
+        row_unit<W, OP, kThreads>(in + base, out + base, 0, R, smem, tab, false);
+    } else {
+        row_unit<W, OP, kThreads>(in, out, row0, R, smem, tab, false);
+    }
 }
 
-template <int H, int MODE>
+template <int H, int MODE, bool G>
 __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, float *out, int w,
+                                                            const size_t *__restrict__ offs,
                                                             const float *__restrict__ tab)
 {
     constexpr int TH = col_threads<H>(), L = col_unit_cols<H, TH>();
@@ -33,65 +44,84 @@ __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, floa
     const int strips = w / L;
     const size_t a = blockIdx.x / strips;
     const int c0 = (int)(blockIdx.x - a * strips) * L;
-    col_unit<H, MODE, TH>(in, out, w, a * (size_t)H * w, c0, L, smem, tab, false);
+    col_unit<H, MODE, TH>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
 }
 
-template <int W>
-int rows_w(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s)
+template <int W, bool G>
+int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
+            int h)
 {
     constexpr int R = row_unit_rows<W, kThreads>();
-    if (nrows % R)
+    if (nrows % R || (G && (h <= 0 || h % R)))
         return -1;
     const dim3 grid((unsigned)(nrows / R)), block(kThreads);
     switch (op) {
-    case FA: hipLaunchKernelGGL((k_rows<W, FA>), grid, block, 0, s, in, out, tab); break;
-    case FA2: hipLaunchKernelGGL((k_rows<W, FA2>), grid, block, 0, s, in, out, tab); break;
-    case IA: hipLaunchKernelGGL((k_rows<W, IA>), grid, block, 0, s, in, out, tab); break;
-    case I0: hipLaunchKernelGGL((k_rows<W, I0>), grid, block, 0, s, in, out, tab); break;
+    case FA: hipLaunchKernelGGL((k_rows<W, FA, G>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case FA2: hipLaunchKernelGGL((k_rows<W, FA2, G>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case IA: hipLaunchKernelGGL((k_rows<W, IA, G>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case I0: hipLaunchKernelGGL((k_rows<W, I0, G>), grid, block, 0, s, in, out, offs, h, tab); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int H>
-int cols_h(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s)
+template <int W>
+int rows_w(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
+           int h)
+{
+    return offs ? rows_wg<W, true>(in, out, nrows, op, tab, s, offs, h)
+                : rows_wg<W, false>(in, out, nrows, op, tab, s, nullptr, 0);
+}
+
+template <int H, bool G>
+int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
+            const size_t *offs)
 {
     constexpr int L = col_unit_cols<H, col_threads<H>()>();
     if (w % L)
         return -1;
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(col_threads<H>());
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_cols<H, 0>), grid, block, 0, s, in, out, w, tab); break;
-    case 1: hipLaunchKernelGGL((k_cols<H, 1>), grid, block, 0, s, in, out, w, tab); break;
-    case 2: hipLaunchKernelGGL((k_cols<H, 2>), grid, block, 0, s, in, out, w, tab); break;
+    case 0: hipLaunchKernelGGL((k_cols<H, 0, G>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 1: hipLaunchKernelGGL((k_cols<H, 1, G>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 2: hipLaunchKernelGGL((k_cols<H, 2, G>), grid, block, 0, s, in, out, w, offs, tab); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int H>
+int cols_h(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
+           const size_t *offs)
+{
+    return offs ? cols_hg<H, true>(w, in, out, batch, mode, tab, s, offs)
+                : cols_hg<H, false>(w, in, out, batch, mode, tab, s, nullptr);
+}
+
 }  // namespace
 
-int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s)
+int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
+                const size_t *offs, int h)
 {
     switch (w) {
-    case 64: return rows_w<64>(in, out, nrows, op, tab, s);
-    case 128: return rows_w<128>(in, out, nrows, op, tab, s);
-    case 256: return rows_w<256>(in, out, nrows, op, tab, s);
-    case 512: return rows_w<512>(in, out, nrows, op, tab, s);
-    case 1024: return rows_w<1024>(in, out, nrows, op, tab, s);
+    case 64: return rows_w<64>(in, out, nrows, op, tab, s, offs, h);
+    case 128: return rows_w<128>(in, out, nrows, op, tab, s, offs, h);
+    case 256: return rows_w<256>(in, out, nrows, op, tab, s, offs, h);
+    case 512: return rows_w<512>(in, out, nrows, op, tab, s, offs, h);
+    case 1024: return rows_w<1024>(in, out, nrows, op, tab, s, offs, h);
     default: return -1;
     }
 }
 
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
-                hipStream_t s)
+                hipStream_t s, const size_t *offs)
 {
     switch (h) {
-    case 64: return cols_h<64>(w, in, out, batch, mode, tab, s);
-    case 128: return cols_h<128>(w, in, out, batch, mode, tab, s);
-    case 256: return cols_h<256>(w, in, out, batch, mode, tab, s);
-    case 512: return cols_h<512>(w, in, out, batch, mode, tab, s);
-    case 1024: return cols_h<1024>(w, in, out, batch, mode, tab, s);
+    case 64: return cols_h<64>(w, in, out, batch, mode, tab, s, offs);
+    case 128: return cols_h<128>(w, in, out, batch, mode, tab, s, offs);
+    case 256: return cols_h<256>(w, in, out, batch, mode, tab, s, offs);
+    case 512: return cols_h<512>(w, in, out, batch, mode, tab, s, offs);
+    case 1024: return cols_h<1024>(w, in, out, batch, mode, tab, s, offs);
     default: return -1;
     }
 }

@@ -15,18 +15,18 @@
 namespace {
 
 int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int pack, int mode,
-                   const float *tab, hipStream_t s)
+                   const float *tab, hipStream_t s, const size_t *offs = nullptr)
 {
     using namespace bdx;
     switch (h) {
-    case 8: return launch_small_h8(w, in, out, batch, pack, mode, tab, s);
-    case 16: return launch_small_h16(w, in, out, batch, pack, mode, tab, s);
-    case 32: return launch_small_h32(w, in, out, batch, pack, mode, tab, s);
-    case 64: return launch_small_h64(w, in, out, batch, pack, mode, tab, s);
-    case 128: return launch_small_h128(w, in, out, batch, pack, mode, tab, s);
-    case 256: return launch_small_h256(w, in, out, batch, pack, mode, tab, s);
-    case 512: return launch_small_h512(w, in, out, batch, pack, mode, tab, s);
-    case 1024: return launch_small_h1024(w, in, out, batch, pack, mode, tab, s);
+    case 8: return launch_small_h8(w, in, out, batch, pack, mode, tab, s, offs);
+    case 16: return launch_small_h16(w, in, out, batch, pack, mode, tab, s, offs);
+    case 32: return launch_small_h32(w, in, out, batch, pack, mode, tab, s, offs);
+    case 64: return launch_small_h64(w, in, out, batch, pack, mode, tab, s, offs);
+    case 128: return launch_small_h128(w, in, out, batch, pack, mode, tab, s, offs);
+    case 256: return launch_small_h256(w, in, out, batch, pack, mode, tab, s, offs);
+    case 512: return launch_small_h512(w, in, out, batch, pack, mode, tab, s, offs);
+    case 1024: return launch_small_h1024(w, in, out, batch, pack, mode, tab, s, offs);
     default: return -1;
     }
 }
@@ -179,24 +179,37 @@ int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t bat
     return bdx::launch_cols(h, w, d_in, d_out, batch, mode, d_tables, s);
 }
 
-int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
-                  const float *d_tables, void *stream)
+static int transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
+                     const float *d_tables, hipStream_t s, const size_t *offs)
 {
     if (batch == 0)
         return 0;
     if (mode < BDX_FORWARD || mode > BDX_ROUNDTRIP)
         return -1;
-    hipStream_t s = (hipStream_t)stream;
     if (bdx::small_route(h, w))
-        return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s);
+        return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s, offs);
     const size_t nrows = batch * (size_t)h;
     const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;
-    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s) ||
-        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s))
+    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s, offs, h) ||
+        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s, offs))
         return -1;
-    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s))
+    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s, offs, h))
         return -1;
     return 0;
+}
+
+int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
+                  const float *d_tables, void *stream)
+{
+    return transform(d_in, d_out, h, w, batch, pack, mode, d_tables, (hipStream_t)stream, nullptr);
+}
+
+int bdx_transform_gather(const float *d_in, float *d_out, int h, int w, size_t count, const size_t *d_offsets,
+                         int pack, int mode, const float *d_tables, void *stream)
+{
+    if (!d_offsets)
+        return -1;
+    return transform(d_in, d_out, h, w, count, pack, mode, d_tables, (hipStream_t)stream, d_offsets);
 }
 
 }  // extern "C"

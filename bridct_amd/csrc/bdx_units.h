@@ -22,9 +22,11 @@ namespace bdx {
 
 template <int H, int W, int THREADS> constexpr int small_per_unit(int lds_floats) { return lds_floats / (H * (W + 1)); }
 
-template <int H, int W, int MODE, int THREADS>
-__device__ __forceinline__ void small_unit(const float *in, float *out, size_t off, int count, float *tile,
-                                           const float *__restrict__ tab)
+// `count` whole arrays; row r of the sequence of their rows (array r / H, row r % H) is read
+// at src(r) and written at dst(r).
+template <int H, int W, int MODE, int THREADS, typename Src, typename Dst>
+__device__ __forceinline__ void small_unit_rows(Src src, Dst dst, int count, float *tile,
+                                                const float *__restrict__ tab)
 {
     constexpr int P = W + 1;
     const auto row_base = [](int L) { return L * P; };
@@ -32,7 +34,7 @@ __device__ __forceinline__ void small_unit(const float *in, float *out, size_t o
         const int a = L / W, c = L - a * W;
         return a * H * P + c;
     };
-    tile_load<W, P, THREADS>(tile, in + off, W, count * H);
+    tile_load_rows<W, P, THREADS>(tile, src, count * H);
     __syncthreads();
     if constexpr (MODE == 0) {
         run_lines<W, 1, FA, THREADS>(tile, count * H, row_base, tab);
@@ -52,7 +54,16 @@ __device__ __forceinline__ void small_unit(const float *in, float *out, size_t o
         run_lines<W, 1, I0, THREADS>(tile, count * H, row_base, tab);
     }
     __syncthreads();
-    tile_store<W, P, THREADS>(tile, out + off, W, count * H);
+    tile_store_rows<W, P, THREADS>(tile, dst, count * H);
+}
+
+// `count` contiguous whole arrays starting at element `off`.
+template <int H, int W, int MODE, int THREADS>
+__device__ __forceinline__ void small_unit(const float *in, float *out, size_t off, int count, float *tile,
+                                           const float *__restrict__ tab)
+{
+    small_unit_rows<H, W, MODE, THREADS>([=](int r) { return in + off + (size_t)r * W; },
+                                         [=](int r) { return out + off + (size_t)r * W; }, count, tile, tab);
 }
 
 // ---- row pass ------------------------------------------------------------------------

@@ -127,6 +127,11 @@ int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
  * per workgroup of the whole-array kernel (0 = automatic; ignored by two-pass shapes). */
 int bdx_transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
                   const float *d_tables, void *stream);
+/* The same transform of `count` h x w arrays at element offsets d_offsets[0, count) of
+ * d_in and d_out (device array; each array 16-byte aligned, arrays disjoint): the arrays
+ * of one shape inside a ragged batch, with the plan path's kernels in gather form. */
+int bdx_transform_gather(const float *d_in, float *d_out, int h, int w, size_t count, const size_t *d_offsets,
+                         int pack, int mode, const float *d_tables, void *stream);
 /* Dense split-MFMA route: operand precision of the hi/lo split. */
 #define BDX_DENSE_F16 1
 #define BDX_DENSE_BF16 2

@@ -102,8 +102,8 @@ BRIDCT_API int bridct_batch_apply_device(bridct_batch *batch, int mode, const fl
  * again whenever (mode, d_input, d_output) changes. */
 BRIDCT_API int bridct_batch_graph_apply(bridct_batch *batch, int mode, const float *d_input,
                                         float *d_output);
-/* Waits for the batch stream; BRIDCT_DEVICE_ERROR if a call since the last one failed on
- * the device (a dependency wait of the persistent kernel that ran out). */
+/* Waits for the batch stream; BRIDCT_DEVICE_ERROR if a call since the last synchronize
+ * failed on the device (a dependency wait of the persistent kernel that ran out). */
 BRIDCT_API int bridct_batch_synchronize(bridct_batch *batch);
 /* Host buffers of bridct_batch_elements floats each (disjoint); synchronous. */
 BRIDCT_API int bridct_batch_apply(bridct_batch *batch, int mode, const float *input, float *output);
@@ -148,6 +148,19 @@ BRIDCT_API int bridct_codec8_decode(bridct_codec8 *codec, const short *coeffs, s
                                     size_t stride, unsigned char *image);
 BRIDCT_API int bridct_codec8_synchronize(bridct_codec8 *codec);
 BRIDCT_API void *bridct_codec8_stream(const bridct_codec8 *codec);
+
+/* Execution strategy of a batch.  PERSISTENT: one persistent kernel launch per direction
+ * (the tiny kernel when every array is at most 32 x 32).  GROUPED: the plan path's kernels
+ * per distinct shape, reading the arrays of that shape through an offset table, one or two
+ * launches per shape on the batch stream (one graph with bridct_batch_graph_apply); its
+ * results are bitwise those of per-shape plans.  AUTO (the default at creation): GROUPED
+ * when the batch averages at least 2^21 pixels per distinct shape and is not tiny, else
+ * PERSISTENT.  get returns the strategy in effect (never AUTO). */
+#define BRIDCT_BATCH_AUTO 0
+#define BRIDCT_BATCH_PERSISTENT 1
+#define BRIDCT_BATCH_GROUPED 2
+BRIDCT_API int bridct_batch_set_strategy(bridct_batch *batch, int strategy);
+BRIDCT_API int bridct_batch_get_strategy(const bridct_batch *batch);
 
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);

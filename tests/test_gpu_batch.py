@@ -1,6 +1,8 @@
-"""GPU tests of ragged batches (one persistent launch over arrays of mixed shapes): each
-array's result equals the plan path of its shape (reference src/plan.c:126-163 applied
-per array) and the FP64 oracle; graph replays equal eager calls."""
+"""GPU tests of ragged batches over arrays of mixed shapes, with both strategies (one
+persistent launch, and the grouped per-shape kernels): each array's result equals the plan
+path of its shape (reference src/plan.c:126-163 applied per array) -- bitwise for the
+grouped strategy, which runs the plan path's kernels -- and the FP64 oracle; graph
+replays equal eager calls."""
 import numpy as np
 import pytest
 
@@ -10,6 +12,15 @@ from bridct_amd import capi
 pytestmark = pytest.mark.gpu
 
 FTI = 3   # BRIDCT_GPU_FORWARD_THEN_INVERSE
+STRATEGIES = [capi.BATCH_PERSISTENT, capi.BATCH_GROUPED]
+
+
+def agree(y, ref, strategy, tol=1e-6):
+    """Grouped results are the plan path's bitwise; persistent ones agree to FP32
+    rounding (some shapes use other kernels there)."""
+    if strategy == capi.BATCH_GROUPED:
+        return np.array_equal(y, ref)
+    return np.abs(y - ref).max() <= tol * np.abs(ref).max()
 
 
 def ragged(shapes, seed):
@@ -51,18 +62,19 @@ def all_shapes():
     return [(h, w) for h in S.LENGTHS for w in S.LENGTHS]
 
 
+@pytest.mark.parametrize('strategy', STRATEGIES)
 @pytest.mark.parametrize('mode', [0, 1, 2, FTI])
-def test_every_shape_once(mode):
+def test_every_shape_once(mode, strategy):
     shapes = all_shapes()
     arrays = ragged(shapes, 11)
     b = capi.Batch(shapes)
     try:
-        print('batch of 64 shapes, mode', mode, flush=True)
+        assert b.set_strategy(strategy) == 0 and b.strategy == strategy
+        print('batch of 64 shapes, mode', mode, 'strategy', strategy, flush=True)
         y = b.split(run_device(b, mode, flat(arrays)))
         ref = plan_results(shapes, arrays, mode)
         for (h, w), a, yi, ri in zip(shapes, arrays, y, ref):
-            scale = np.abs(ri).max()
-            assert np.abs(yi - ri).max() <= 1e-6 * scale, (h, w)
+            assert agree(yi, ri, strategy), (h, w)
             if mode in (0, 1):
                 o = S.oracle_2d(a, mode)
                 assert S.max_rel(yi, o) < 1e-6, (h, w)
@@ -72,19 +84,21 @@ def test_every_shape_once(mode):
         b.close()
 
 
-def test_ragged_seeded_mix():
+@pytest.mark.parametrize('strategy', STRATEGIES)
+def test_ragged_seeded_mix(strategy):
     rng = np.random.default_rng(7)
     shapes = [(int(rng.choice(S.LENGTHS)), int(rng.choice(S.LENGTHS))) for _ in range(400)]
     arrays = ragged(shapes, 7)
     b = capi.Batch(shapes)
     try:
+        assert b.set_strategy(strategy) == 0
         x = flat(arrays)
         for mode in (0, 1, 2):
-            print('ragged 400 arrays, mode', mode, flush=True)
+            print('ragged 400 arrays, mode', mode, 'strategy', strategy, flush=True)
             y = b.split(run_device(b, mode, x))
             ref = plan_results(shapes, arrays, mode)
             for (h, w), yi, ri in zip(shapes, y, ref):
-                assert np.abs(yi - ri).max() <= 1e-6 * np.abs(ri).max(), (h, w)
+                assert agree(yi, ri, strategy), (h, w)
         print('ragged 400 arrays, in place', flush=True)
         y_in_place = run_device(b, 0, x, in_place=True)
         assert np.array_equal(y_in_place, run_device(b, 0, x))
@@ -133,11 +147,13 @@ def test_tiny_kernel_mix():
         b.close()
 
 
-def test_large_arrays_with_small_fillers():
+@pytest.mark.parametrize('strategy', STRATEGIES)
+def test_large_arrays_with_small_fillers(strategy):
     shapes = [(1024, 1024)] * 3 + [(16, 16)] * 500 + [(512, 1024), (8, 1024), (1024, 8)]
     arrays = ragged(shapes, 5)
     b = capi.Batch(shapes)
     try:
+        assert b.set_strategy(strategy) == 0
         for mode in (2, FTI):
             print('large + fillers, mode', mode, flush=True)
             y = b.split(run_device(b, mode, flat(arrays)))
@@ -147,7 +163,8 @@ def test_large_arrays_with_small_fillers():
         b.close()
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize('strategy', STRATEGIES)
+def test_graph_replays_match_eager(strategy):
     rng = np.random.default_rng(9)
     shapes = [(int(rng.choice(S.LENGTHS)), int(rng.choice(S.LENGTHS))) for _ in range(60)]
     arrays = ragged(shapes, 9)
@@ -156,6 +173,7 @@ def test_graph_replays_match_eager():
     d_in = capi.DeviceBuffer(x.nbytes)
     d_out = capi.DeviceBuffer(x.nbytes)
     try:
+        assert b.set_strategy(strategy) == 0
         d_in.upload(x)
         eager = {}
         for mode in (0, FTI):
@@ -180,18 +198,59 @@ def test_graph_replays_match_eager():
         b.close()
 
 
-def test_host_apply_matches_device():
+@pytest.mark.parametrize('strategy', STRATEGIES)
+def test_host_apply_matches_device(strategy):
     shapes = [(64, 256), (256, 64), (8, 8), (1024, 32), (128, 128)]
     arrays = ragged(shapes, 2)
     x = flat(arrays)
     b = capi.Batch(shapes)
     try:
+        assert b.set_strategy(strategy) == 0
         for mode in (0, 1, 2, FTI):
             y = np.empty_like(x)
             assert b.apply(mode, x, y) == 0
             assert np.array_equal(y, run_device(b, mode, x))
     finally:
         b.close()
+
+
+def test_strategy_selection_and_switching():
+    """AUTO resolves to GROUPED for large mixes (at least 2^21 pixels per distinct shape on
+    average) and to PERSISTENT otherwise; switching strategies between graph replays
+    recaptures the graph; a tiny batch also runs grouped."""
+    big = capi.Batch([(1024, 1024)] * 4 + [(512, 256)] * 20)
+    small = capi.Batch([(64, 64), (128, 32)] * 10)
+    tiny = capi.Batch([(8, 8), (16, 32)] * 100)
+    try:
+        assert big.strategy == capi.BATCH_GROUPED
+        assert small.strategy == capi.BATCH_PERSISTENT and tiny.strategy == capi.BATCH_PERSISTENT
+        for bad in (-1, 3):
+            assert big.set_strategy(bad) == capi.INVALID_ARGUMENT
+        assert big.set_strategy(capi.BATCH_PERSISTENT) == 0 and big.strategy == capi.BATCH_PERSISTENT
+        assert big.set_strategy(capi.BATCH_AUTO) == 0 and big.strategy == capi.BATCH_GROUPED
+        x = flat(ragged(big.shapes, 21))
+        d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+        d_in.upload(x)
+        outs = []
+        for strategy in (capi.BATCH_GROUPED, capi.BATCH_PERSISTENT, capi.BATCH_GROUPED):
+            print('graph with strategy', strategy, flush=True)
+            assert big.set_strategy(strategy) == 0
+            assert big.graph_apply(0, d_in.ptr, d_out.ptr) == 0
+            assert big.synchronize() == 0
+            y = np.empty_like(x)
+            d_out.download(y)
+            outs.append(y)
+        ref = flat(plan_results(big.shapes, big.split(x), 0))
+        assert np.array_equal(outs[0], ref) and np.array_equal(outs[2], ref)
+        assert np.abs(outs[1] - ref).max() <= 1e-6 * np.abs(ref).max()
+        xt = flat(ragged(tiny.shapes, 22))
+        assert tiny.set_strategy(capi.BATCH_GROUPED) == 0
+        yt = run_device(tiny, 0, xt)
+        assert np.array_equal(yt, flat(plan_results(tiny.shapes, tiny.split(xt), 0)))
+    finally:
+        big.close()
+        small.close()
+        tiny.close()
 
 
 def test_batch_argument_errors():
