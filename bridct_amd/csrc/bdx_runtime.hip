@@ -65,6 +65,40 @@ int bdx_free(void *ptr)
     return hipFree(ptr) == hipSuccess ? 0 : -1;
 }
 
+int bdx_host_malloc(void **ptr, size_t bytes)
+{
+    return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? 0 : -1;
+}
+
+int bdx_host_free(void *ptr)
+{
+    return hipHostFree(ptr) == hipSuccess ? 0 : -1;
+}
+
+int bdx_event_create(void **event)
+{
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return -1;
+    *event = (void *)e;
+    return 0;
+}
+
+int bdx_event_destroy(void *event)
+{
+    return hipEventDestroy((hipEvent_t)event) == hipSuccess ? 0 : -1;
+}
+
+int bdx_event_record(void *event, void *stream)
+{
+    return hipEventRecord((hipEvent_t)event, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int bdx_event_sync(void *event)
+{
+    return hipEventSynchronize((hipEvent_t)event) == hipSuccess ? 0 : -1;
+}
+
 int bdx_stream_create(void **stream)
 {
     hipStream_t s;

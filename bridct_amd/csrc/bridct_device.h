@@ -115,6 +115,12 @@ int bdx_graph_destroy(void *graph_exec);
 int bdx_device_ready(void);
 int bdx_malloc(void **ptr, size_t bytes);
 int bdx_free(void *ptr);
+int bdx_host_malloc(void **ptr, size_t bytes);   /* pinned host memory */
+int bdx_host_free(void *ptr);
+int bdx_event_create(void **event);
+int bdx_event_destroy(void *event);
+int bdx_event_record(void *event, void *stream);
+int bdx_event_sync(void *event);
 int bdx_stream_create(void **stream);
 int bdx_stream_destroy(void *stream);
 int bdx_stream_sync(void *stream);

@@ -27,6 +27,14 @@ static const double PI = 3.14159265358979323846;
 /* Floats of staging per direction (host-pointer calls are copied in chunks of arrays). */
 #define STAGE_FLOATS ((size_t)1 << 20)
 
+/* Host calls of at least this many floats go through the pinned pipeline: two chunk slots,
+ * each with a pinned input and output buffer, a device input and output buffer and an
+ * event.  The host copies chunk k into pinned memory while the GPU copies in, transforms
+ * and copies out chunk k - 1, then copies finished chunks out of pinned memory.  Pageable
+ * copies run at 31 GB/s (in) and 18 GB/s (out) here; pinned ones at 57 GB/s each way, so
+ * the host memcpy becomes the bound. */
+#define PIPELINE_MIN_FLOATS ((size_t)1 << 21)
+
 struct bridct_plan {
     int height, width;
     size_t elements;      /* h * w */
@@ -34,6 +42,9 @@ struct bridct_plan {
     void *stream;         /* hipStream_t */
     float *d_tables;      /* BDX_TABLE_FLOATS */
     float *d_in, *d_out;  /* chunk * elements each */
+    /* pinned pipeline for large host calls, allocated on first use */
+    float *pin_in[2], *pin_out[2], *d_in2, *d_out2;
+    void *ev[2];
     int route;            /* BRIDCT_ROUTE_* */
     int pack;             /* arrays per workgroup of the whole-array kernel, 0 = automatic */
     float *d_dense;       /* dense route: Q_h', Q_h'^T, Q_w' */
@@ -130,6 +141,18 @@ void bridct_plan_destroy(bridct_plan *plan)
         bdx_free(plan->d_in);
     if (plan->d_out)
         bdx_free(plan->d_out);
+    if (plan->d_in2)
+        bdx_free(plan->d_in2);
+    if (plan->d_out2)
+        bdx_free(plan->d_out2);
+    for (int i = 0; i < 2; i++) {
+        if (plan->pin_in[i])
+            bdx_host_free(plan->pin_in[i]);
+        if (plan->pin_out[i])
+            bdx_host_free(plan->pin_out[i]);
+        if (plan->ev[i])
+            bdx_event_destroy(plan->ev[i]);
+    }
     if (plan->stream)
         bdx_stream_destroy(plan->stream);
     free(plan);
@@ -324,8 +347,59 @@ static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_ou
                          plan->stream);
 }
 
+static int pipeline_ready(bridct_plan *plan)
+{
+    if (plan->ev[1])
+        return 0;
+    const size_t bytes = plan->chunk * plan->elements * sizeof(float);
+    for (int i = 0; i < 2; i++)
+        if ((!plan->pin_in[i] && bdx_host_malloc((void **)&plan->pin_in[i], bytes)) ||
+            (!plan->pin_out[i] && bdx_host_malloc((void **)&plan->pin_out[i], bytes)))
+            return -1;
+    if ((!plan->d_in2 && bdx_malloc((void **)&plan->d_in2, bytes)) ||
+        (!plan->d_out2 && bdx_malloc((void **)&plan->d_out2, bytes)) ||
+        (!plan->ev[0] && bdx_event_create(&plan->ev[0])) || bdx_event_create(&plan->ev[1]))
+        return -1;
+    return 0;
+}
+
+/* Large host calls: chunks alternate between two slots (see PIPELINE_MIN_FLOATS). */
+static int run_host_pipelined(bridct_plan *plan, int mode, const float *input, float *output, size_t batch)
+{
+    const size_t per = plan->chunk, nchunks = (batch + per - 1) / per;
+    float *d_in[2] = {plan->d_in, plan->d_in2}, *d_out[2] = {plan->d_out, plan->d_out2};
+    size_t floats_of[2] = {0, 0};
+    int rc = BRIDCT_OK;
+    for (size_t k = 0; k < nchunks + 2 && rc == BRIDCT_OK; k++) {
+        const int s = (int)(k & 1);
+        if (k >= 2) {   /* chunk k - 2 finished in slot s: copy it out */
+            const size_t j = k - 2;
+            if (bdx_event_sync(plan->ev[s]))
+                rc = BRIDCT_DEVICE_ERROR;
+            else
+                memcpy(output + j * per * plan->elements, plan->pin_out[s], floats_of[s] * sizeof(float));
+        }
+        if (k >= nchunks || rc != BRIDCT_OK)
+            continue;
+        const size_t count = batch - k * per < per ? batch - k * per : per;
+        const size_t floats = count * plan->elements;
+        floats_of[s] = floats;
+        memcpy(plan->pin_in[s], input + k * per * plan->elements, floats * sizeof(float));
+        if (bdx_copy_h2d(d_in[s], plan->pin_in[s], floats * sizeof(float), plan->stream) ||
+            transform(plan, mode, d_in[s], d_out[s], count) ||
+            bdx_copy_d2h(plan->pin_out[s], d_out[s], floats * sizeof(float), plan->stream) ||
+            bdx_event_record(plan->ev[s], plan->stream))
+            rc = BRIDCT_DEVICE_ERROR;
+    }
+    if (bdx_stream_sync(plan->stream))
+        rc = BRIDCT_DEVICE_ERROR;
+    return rc;
+}
+
 static int run_host(bridct_plan *plan, int mode, const float *input, float *output, size_t batch)
 {
+    if (batch > plan->chunk && batch * plan->elements >= PIPELINE_MIN_FLOATS && pipeline_ready(plan) == 0)
+        return run_host_pipelined(plan, mode, input, output, batch);
     for (size_t done = 0; done < batch;) {
         size_t count = batch - done;
         if (count > plan->chunk)
