@@ -25,7 +25,10 @@
  * is specialized for its shape (registers, LDS, team size) and needs no cross-workgroup
  * dependency waits, which makes it faster when every shape has enough pixels to fill the
  * GPU; the automatic choice takes it when the average shape has at least
- * GROUPED_MIN_PIXELS pixels.  Batches of tiny arrays always use the tiny kernel.
+ * GROUPED_MIN_PIXELS pixels.  Batches of tiny arrays always use the tiny kernel.  The
+ * whole-array shapes, whose launches are short and leave most of the GPU idle, go on a
+ * second stream (forked from and joined back to the batch stream with events, so a
+ * graph capture records both) and run beside the two-pass shapes.
  *
  * The object owns its stream, device tables, item lists and counters, the grouped offset
  * table, and one captured graph (hipGraph) for replays of the same (mode, input, output);
@@ -72,6 +75,8 @@ struct bridct_batch {
     group *groups;           /* distinct shapes, largest total first */
     unsigned ngroups;
     size_t *d_goffs;         /* element offsets of the arrays, grouped by shape */
+    void *aux_stream;        /* whole-array shapes of the grouped strategy */
+    void *ev_fork, *ev_join;
     void *stream;
     float *d_tables;
     bdx_run *d_runs;
@@ -202,6 +207,14 @@ void bridct_batch_destroy(bridct_batch *b)
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             bdx_free(bufs[i]);
+    if (b->aux_stream) {
+        bdx_stream_sync(b->aux_stream);
+        bdx_stream_destroy(b->aux_stream);
+    }
+    if (b->ev_fork)
+        bdx_event_destroy(b->ev_fork);
+    if (b->ev_join)
+        bdx_event_destroy(b->ev_join);
     if (b->stream)
         bdx_stream_destroy(b->stream);
     free(b->offsets);
@@ -395,7 +408,9 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         upload((void **)&b->d_items[0], sc.items[0], sc.nitems[0] * sizeof(bdx_item), b->stream) ||
         upload((void **)&b->d_items[1], sc.items[1], sc.nitems[1] * sizeof(bdx_item), b->stream) ||
         upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) ||
-        upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) || bdx_stream_sync(b->stream))
+        upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) ||
+        bdx_stream_create(&b->aux_stream) || bdx_event_create(&b->ev_fork) || bdx_event_create(&b->ev_join) ||
+        bdx_stream_sync(b->stream))
         goto fail;
     free(host_tables);
     free(zeros);
@@ -436,15 +451,25 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
                             mode, b->d_tables, b->blocks, b->spin_limit, b->tiny, b->stream);
 }
 
-/* GROUPED: the plan path's kernels, per distinct shape, in gather form. */
+/* GROUPED: the plan path's kernels, per distinct shape, in gather form; two-pass shapes
+ * on the batch stream, whole-array shapes on the auxiliary stream beside them. */
 static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
+    unsigned nsmall = 0;
+    for (unsigned i = 0; i < b->ngroups; i++)
+        nsmall += BDX_SMALL_ROUTE(b->groups[i].h, b->groups[i].w);
+    const int fork = nsmall > 0 && nsmall < b->ngroups;
+    if (fork && (bdx_event_record(b->ev_fork, b->stream) || bdx_stream_wait_event(b->aux_stream, b->ev_fork)))
+        return -1;
     for (unsigned i = 0; i < b->ngroups; i++) {
         const group *g = &b->groups[i];
+        void *s = fork && BDX_SMALL_ROUTE(g->h, g->w) ? b->aux_stream : b->stream;
         if (bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
-                                 b->d_tables, b->stream))
+                                 b->d_tables, s))
             return -1;
     }
+    if (fork && (bdx_event_record(b->ev_join, b->aux_stream) || bdx_stream_wait_event(b->stream, b->ev_join)))
+        return -1;
     return 0;
 }
 

@@ -12,7 +12,7 @@ enum Op { FA = 1, FA2 = 2, IA = 3, I0 = 4 };
 
 constexpr int kThreads = 256;
 // Floats of LDS for one workgroup of the whole-array kernel (144 KiB of the 160 KiB).
-constexpr int kSmallLds = 36864;
+constexpr int kSmallLds = BDX_SMALL_LDS_FLOATS;
 // Floats of LDS per workgroup of the row / column pass kernels (two workgroups per CU).
 constexpr int kPassLds = 16384;
 

@@ -99,6 +99,11 @@ int bdx_event_sync(void *event)
     return hipEventSynchronize((hipEvent_t)event) == hipSuccess ? 0 : -1;
 }
 
+int bdx_stream_wait_event(void *stream, void *event)
+{
+    return hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0) == hipSuccess ? 0 : -1;
+}
+
 int bdx_stream_create(void **stream)
 {
     hipStream_t s;

@@ -43,6 +43,11 @@ extern "C" {
 #define BDX_ROW_UNIT(w) (BDX_ROW_BAND(w) * ((w) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_COL_UNIT(h) (BDX_COL_STRIP(h) * ((h) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_CEIL(a, b) (((a) + (b) - 1) / (b))
+/* Whole-array route of the plan path: an h x w array with h (w + 1) <= BDX_SMALL_LDS_FLOATS
+ * is transformed by one workgroup in LDS. */
+#define BDX_SMALL_LDS_FLOATS 36864
+#define BDX_SMALL_ROUTE(h, w) ((long)(h) * ((w) + 1) <= BDX_SMALL_LDS_FLOATS)
+
 /* Batches whose arrays all have both axes at most BDX_TINY_MAX have no dependent items: they
  * run the tiny kernel instead of the persistent queue, one workgroup per item, about
  * BDX_TINY_PIXELS pixels per item (the whole-array packing that measured fastest for these
@@ -121,6 +126,7 @@ int bdx_event_create(void **event);
 int bdx_event_destroy(void *event);
 int bdx_event_record(void *event, void *stream);
 int bdx_event_sync(void *event);
+int bdx_stream_wait_event(void *stream, void *event);   /* later work on stream waits for event */
 int bdx_stream_create(void **stream);
 int bdx_stream_destroy(void *stream);
 int bdx_stream_sync(void *stream);
