@@ -14,7 +14,8 @@
 //
 // Kernel: 256 lanes, a 64 x 64 or 128 x 128 output tile per workgroup, four waves, k steps
 // of 32 staged in LDS as hi/lo halves ([row][k] for A, [column][k] for B, so that every
-// lane reads its eight k values of a v_mfma_f32_32x32x16 operand with one 16-byte load).
+// lane reads its eight k values of a v_mfma_f32_32x32x16 operand with one 16-byte load);
+// the next step's FP32 operands are loaded into registers during the current products.
 // MFMA lane layout (checked on gfx950): lane l holds A[l % 32][8 (l / 32) + i],
 // B[8 (l / 32) + i][l % 32], and result element a at row 8 (a / 4) + 4 (l / 32) + a % 4,
 // column l % 32.
@@ -102,37 +103,49 @@ __global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *
     f16v acc[S][S] = {}, acc_lo[S][S] = {};   // hi x hi products, and the two small cross products
     const int ar = tid >> 2, ak = (tid & 3) * 8;            // A / B^T: row, first k
     const int bn = tid & 63, bk = (tid >> 6) * 8;           // B: column, first k
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    // The next k step's operands are loaded into registers before the current step's
+    // products, so that their latency hides behind the MFMAs.
+    float xa[S][8], xb[S][8];
+    const auto load = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < S; j++) {
-            float x[8];
             const float *src = A + (long long)(ar + 64 * j) * K + k0 + ak;
             const float4 u = *reinterpret_cast<const float4 *>(src);
             const float4 v = *reinterpret_cast<const float4 *>(src + 4);
-            x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
-            const int o = (ar + 64 * j) * PITCH + ak;
-            stage8(x, reinterpret_cast<V *>(a_hi + o), reinterpret_cast<V *>(a_lo + o));
+            xa[j][0] = u.x, xa[j][1] = u.y, xa[j][2] = u.z, xa[j][3] = u.w;
+            xa[j][4] = v.x, xa[j][5] = v.y, xa[j][6] = v.z, xa[j][7] = v.w;
         }
 #pragma unroll
         for (int j = 0; j < S; j++) {
-            float x[8];
             if constexpr (BT) {
                 const float *src = B + (n0 + ar + 64 * j) * K + k0 + ak;
                 const float4 u = *reinterpret_cast<const float4 *>(src);
                 const float4 v = *reinterpret_cast<const float4 *>(src + 4);
-                x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
-                const int o = (ar + 64 * j) * PITCH + ak;
-                stage8(x, reinterpret_cast<V *>(b_hi + o), reinterpret_cast<V *>(b_lo + o));
+                xb[j][0] = u.x, xb[j][1] = u.y, xb[j][2] = u.z, xb[j][3] = u.w;
+                xb[j][4] = v.x, xb[j][5] = v.y, xb[j][6] = v.z, xb[j][7] = v.w;
             } else {
                 const float *src = B + (long long)(k0 + bk) * N + n0 + bn + 64 * j;
 #pragma unroll
                 for (int i = 0; i < 8; i++)
-                    x[i] = src[(long long)i * N];
-                const int o = (bn + 64 * j) * PITCH + bk;
-                stage8(x, reinterpret_cast<V *>(b_hi + o), reinterpret_cast<V *>(b_lo + o));
+                    xb[j][i] = src[(long long)i * N];
             }
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const int o = (ar + 64 * j) * PITCH + ak;
+            stage8(xa[j], reinterpret_cast<V *>(a_hi + o), reinterpret_cast<V *>(a_lo + o));
+        }
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const int o = BT ? (ar + 64 * j) * PITCH + ak : (bn + 64 * j) * PITCH + bk;
+            stage8(xb[j], reinterpret_cast<V *>(b_hi + o), reinterpret_cast<V *>(b_lo + o));
+        }
         __syncthreads();
+        if (k0 + BK < K)
+            load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             V ah[S], al[S], bh[S], bl[S];
