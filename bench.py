@@ -140,6 +140,24 @@ def cpu_baseline(inputs):
     if rc:
         return None
     gpx = data.size / ns.value
+    ns_1 = ctypes.c_double()
+    gpx_1 = None
+    if lib.ref_bench_run(len(hs), h_a.ctypes.data, w_a.ctypes.data, o_a.ctypes.data, data.ctypes.data,
+                         out.ctypes.data, spec.ctypes.data, 3, 1, 5, 200.0, ctypes.byref(ns_1)) == 0:
+        gpx_1 = round(data.size / ns_1.value, 4)
+    lib.ref_bench_avx2_fma.restype = ctypes.c_int
+    avx2 = bool(lib.ref_bench_avx2_fma())
+    ref = ctypes.CDLL(os.path.join(ROOT, 'oracle', '_ref', 'libbridct.so'))
+    ref.bridct_plan_create.restype = ctypes.c_void_p
+    ref.bridct_plan_create.argtypes = [ctypes.c_int, ctypes.c_int]
+    ref.bridct_plan_route.restype = ctypes.c_char_p
+    ref.bridct_plan_route.argtypes = [ctypes.c_void_p]
+    ref.bridct_plan_destroy.argtypes = [ctypes.c_void_p]
+    routes = {}
+    for h, w in list(inputs) + [(256, 256)]:
+        p = ref.bridct_plan_create(h, w)
+        routes['%dx%d' % (h, w)] = ref.bridct_plan_route(p).decode() if p else None
+        ref.bridct_plan_destroy(p)
     # the plain 256x256 method: 256 arrays (seeds 0..255), forward then inverse
     c1 = c1_inputs()
     n1 = c1.shape[0]
@@ -163,7 +181,8 @@ def cpu_baseline(inputs):
     return {'value': round(gpx, 4), 'unit': 'Gpixel/s', 'cores': threads, 'kind': 'reference',
             'sample': '%d arrays of each shape (%d Mpx), forward then inverse, one plan per '
                       'thread, median of 5 blocks; host CPU %s' % (per_shape, data.size // 10 ** 6, cpu),
-            'plain_256x256_gpx_s': c1_rates}
+            'plain_256x256_gpx_s': c1_rates, 'value_threads_1': gpx_1, 'host_avx2_fma': avx2,
+            'reference_routes': routes}
 
 
 KERNELS_1024 = (

@@ -176,3 +176,14 @@ int ref_bench_run(int count, const int *h, const int *w, const size_t *offset,
     free(jobs);
     return rc;
 }
+
+/* 1 when this host has AVX2 and FMA, i.e. when the reference library, built with its
+ * runtime-selected AVX2 objects, takes its AVX2 routes (src/bridct_runtime.h:16-32). */
+int ref_bench_avx2_fma(void)
+{
+#if defined(__x86_64__)
+    return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+#else
+    return 0;
+#endif
+}
