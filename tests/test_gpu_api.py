@@ -143,6 +143,36 @@ def test_device_batch_inplace_and_host_agree(L, h, w):
         plan.close()
 
 
+@pytest.mark.parametrize('h,w,batch', [(64, 64, 600), (256, 256, 37), (1024, 1024, 3), (8, 16, 40000)])
+def test_host_pipeline_chunks(L, h, w, batch):
+    """Host calls of 2^18 floats or more go through the two-slot pinned pipeline in chunks
+    of 2^20 floats: several chunks with a partial last one, a single partial chunk, and one
+    array per chunk all equal the device path bitwise, and the input is left unchanged."""
+    rng = np.random.default_rng(batch + h)
+    x = rng.uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    x0 = x.copy()
+    plan = capi.Plan(h, w)
+    d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    try:
+        d_in.upload(x)
+        for mode in (0, 1, capi.FORWARD_THEN_INVERSE):
+            print('pipeline %dx%d batch %d mode %d' % (h, w, batch, mode), flush=True)
+            host = np.full_like(x, np.nan)
+            assert plan.apply_batch(mode, x, host) == 0
+            if mode == capi.FORWARD_THEN_INVERSE:
+                assert plan.apply_device(0, d_in.ptr, d_out.ptr, batch) == 0
+                assert plan.apply_device(1, d_out.ptr, d_out.ptr, batch) == 0
+            else:
+                assert plan.apply_device(mode, d_in.ptr, d_out.ptr, batch) == 0
+            assert plan.synchronize() == 0
+            dev = np.empty_like(x)
+            d_out.download(dev)
+            assert np.array_equal(dev, host)
+            assert np.array_equal(x, x0)
+    finally:
+        plan.close()
+
+
 def test_forward_then_inverse_mode_equals_two_calls(L):
     rng = np.random.default_rng(3)
     for h, w in [(16, 16), (128, 512), (1024, 1024)]:
