@@ -199,18 +199,19 @@ void bridct_batch_destroy(bridct_batch *b)
 {
     if (!b)
         return;
+    /* both streams drained before any buffer they may still read is freed */
     if (b->stream)
         bdx_stream_sync(b->stream);
+    if (b->aux_stream)
+        bdx_stream_sync(b->aux_stream);
     if (b->graph)
         bdx_graph_destroy(b->graph);
     void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out, b->d_goffs};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             bdx_free(bufs[i]);
-    if (b->aux_stream) {
-        bdx_stream_sync(b->aux_stream);
+    if (b->aux_stream)
         bdx_stream_destroy(b->aux_stream);
-    }
     if (b->ev_fork)
         bdx_event_destroy(b->ev_fork);
     if (b->ev_join)
@@ -461,16 +462,17 @@ static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d
     const int fork = nsmall > 0 && nsmall < b->ngroups;
     if (fork && (bdx_event_record(b->ev_fork, b->stream) || bdx_stream_wait_event(b->aux_stream, b->ev_fork)))
         return -1;
-    for (unsigned i = 0; i < b->ngroups; i++) {
+    int rc = 0;
+    for (unsigned i = 0; i < b->ngroups && !rc; i++) {
         const group *g = &b->groups[i];
         void *s = fork && BDX_SMALL_ROUTE(g->h, g->w) ? b->aux_stream : b->stream;
-        if (bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
-                                 b->d_tables, s))
-            return -1;
+        rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
+                                  b->d_tables, s);
     }
+    /* joined even after a failed launch, so that the batch stream covers everything issued */
     if (fork && (bdx_event_record(b->ev_join, b->aux_stream) || bdx_stream_wait_event(b->stream, b->ev_join)))
-        return -1;
-    return 0;
+        rc = -1;
+    return rc;
 }
 
 static int launch_any(bridct_batch *b, int mode, const float *d_in, float *d_out)
@@ -542,7 +544,7 @@ int bridct_batch_synchronize(bridct_batch *b)
     if (!b)
         return BRIDCT_INVALID_ARGUMENT;
     unsigned err = 0;
-    if (bdx_stream_sync(b->stream) ||
+    if (bdx_stream_sync(b->stream) || bdx_stream_sync(b->aux_stream) ||
         bdx_copy_d2h(&err, b->d_ctr + BDX_CTR_ERROR, sizeof(err), b->stream) || bdx_stream_sync(b->stream))
         return BRIDCT_DEVICE_ERROR;
     if (err) {   /* reported once: grouped calls do not reset the flag themselves */
