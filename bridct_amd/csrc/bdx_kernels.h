@@ -11,6 +11,8 @@ namespace bdx {
 enum Op { FA = 1, FA2 = 2, IA = 3, I0 = 4 };
 
 constexpr int kThreads = 256;
+// Accelerator complex dies (XCDs) of an MI355X; each has its own L2.
+constexpr unsigned kXcds = 8;
 // Floats of LDS for one workgroup of the whole-array kernel (144 KiB of the 160 KiB).
 constexpr int kSmallLds = BDX_SMALL_LDS_FLOATS;
 // Floats of LDS per workgroup of the row / column pass kernels (two workgroups per CU).

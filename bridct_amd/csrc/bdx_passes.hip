@@ -42,8 +42,14 @@ __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, floa
     constexpr int TH = col_threads<H>(), L = col_unit_cols<H, TH>();
     __shared__ float smem[col_unit_lds<H, TH>()];
     const int strips = w / L;
-    const size_t a = blockIdx.x / strips;
-    const int c0 = (int)(blockIdx.x - a * strips) * L;
+    // Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD b % kXcds);
+    // the strips are renumbered so that each XCD takes one contiguous range of them
+    // (measured: 1024^2 transforms 5% faster than with consecutive strips on alternating
+    // XCDs).  A permutation of [0, gridDim.x) when the grid is a multiple of kXcds.
+    const unsigned nb = gridDim.x;
+    const unsigned bid = nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+    const size_t a = bid / strips;
+    const int c0 = (int)(bid - a * strips) * L;
     col_unit<H, MODE, TH>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
 }
 
