@@ -448,8 +448,11 @@ size_t bridct_batch_items(const bridct_batch *b, int mode)
 static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
     const int fused = mode == BDX_ROUNDTRIP;
+    int flags = 0;
+    if (b->tiny)
+        flags = BDX_LAUNCH_TINY | (2 * b->elements * sizeof(float) > BDX_STREAM_MIN_BYTES ? BDX_LAUNCH_STREAM : 0);
     return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_ctr, b->nctr,
-                            mode, b->d_tables, b->blocks, b->spin_limit, b->tiny, b->stream);
+                            mode, b->d_tables, b->blocks, b->spin_limit, flags, b->stream);
 }
 
 /* GROUPED: the plan path's kernels, per distinct shape, in gather form; two-pass shapes

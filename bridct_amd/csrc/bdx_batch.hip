@@ -215,19 +215,21 @@ __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, co
 
 // ---- tiny kernel: whole arrays with both axes <= BDX_TINY_MAX only -----------------------
 
-template <int H> __device__ __forceinline__ void tiny_h(int w, const float *in, float *out, size_t off, int count,
-                                                        float *smem, const float *__restrict__ tab)
+template <int H, bool NT>
+__device__ __forceinline__ void tiny_h(int w, const float *in, float *out, size_t off, int count, float *smem,
+                                       const float *__restrict__ tab)
 {
     switch (w) {
-    case 8: small_unit<H, 8, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
-    case 16: small_unit<H, 16, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
-    default: small_unit<H, 32, BDX_MODE, TH>(in, out, off, count, smem, tab); break;
+    case 8: small_unit<H, 8, BDX_MODE, TH, NT>(in, out, off, count, smem, tab); break;
+    case 16: small_unit<H, 16, BDX_MODE, TH, NT>(in, out, off, count, smem, tab); break;
+    default: small_unit<H, 32, BDX_MODE, TH, NT>(in, out, off, count, smem, tab); break;
     }
 }
 
 // Every item of a tiny batch is a SMALL item (the host checked the shapes), and no item
 // depends on another: one workgroup per item, in a plain grid the hardware schedules, with
 // the units inlined and a small LDS area, so that many workgroups stay resident per CU.
+template <bool NT>
 __global__ __launch_bounds__(TH) void k_batch_tiny(const float *in, float *out, const bdx_run *__restrict__ runs,
                                                    const bdx_item *__restrict__ items, const float *__restrict__ tab)
 {
@@ -240,9 +242,9 @@ __global__ __launch_bounds__(TH) void k_batch_tiny(const float *in, float *out, 
     const int h = (int)rn.h, w = (int)rn.w;
     const size_t base = rn.off + (size_t)it.idx * h * w;
     switch (h) {
-    case 8: tiny_h<8>(w, in, out, base, (int)it.unit, smem, tab); break;
-    case 16: tiny_h<16>(w, in, out, base, (int)it.unit, smem, tab); break;
-    default: tiny_h<32>(w, in, out, base, (int)it.unit, smem, tab); break;
+    case 8: tiny_h<8, NT>(w, in, out, base, (int)it.unit, smem, tab); break;
+    case 16: tiny_h<16, NT>(w, in, out, base, (int)it.unit, smem, tab); break;
+    default: tiny_h<32, NT>(w, in, out, base, (int)it.unit, smem, tab); break;
     }
 }
 
@@ -251,12 +253,14 @@ __global__ __launch_bounds__(TH) void k_batch_tiny(const float *in, float *out, 
 #define BDX_CAT2(a, b) a##b
 #define BDX_CAT(a, b) BDX_CAT2(a, b)
 
-int BDX_CAT(launch_batch_m, BDX_MODE)(int tiny, const float *in, float *out, const bdx_run *runs,
+int BDX_CAT(launch_batch_m, BDX_MODE)(int flags, const float *in, float *out, const bdx_run *runs,
                                       const bdx_item *items, unsigned nitems, unsigned *ctr, const float *tab,
                                       unsigned blocks, unsigned spin_limit, hipStream_t s)
 {
-    if (tiny)
-        hipLaunchKernelGGL(k_batch_tiny, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
+    if ((flags & BDX_LAUNCH_TINY) && (flags & BDX_LAUNCH_STREAM))
+        hipLaunchKernelGGL(k_batch_tiny<true>, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
+    else if (flags & BDX_LAUNCH_TINY)
+        hipLaunchKernelGGL(k_batch_tiny<false>, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
     else
         hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, nitems, ctr, tab,
                            spin_limit);

@@ -47,19 +47,20 @@ int launch_small_h1024(int w, const float *in, float *out, size_t batch, int pac
 
 // Row pass over nrows rows of length w: op 1 = forward*alpha, 2 = forward*alpha^2,
 // 3 = alpha*inverse, 4 = inverse.
+// nt: non-temporal (streaming) loads and stores of the array data.
 int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
-                const size_t *offs = nullptr, int h = 0);
+                const size_t *offs = nullptr, int h = 0, bool nt = false);
 // Column pass over `batch` h x w arrays, in place allowed: mode 0 forward*alpha,
 // 1 alpha*inverse, 2 forward*alpha^2 then inverse.
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
-                hipStream_t s, const size_t *offs = nullptr);
+                hipStream_t s, const size_t *offs = nullptr, bool nt = false);
 
-// Persistent batch kernel, one per mode (bdx_batch.hip).
-int launch_batch_m0(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+// Persistent batch kernel, one per mode (bdx_batch.hip); flags: BDX_LAUNCH_*.
+int launch_batch_m0(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
-int launch_batch_m1(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+int launch_batch_m1(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
-int launch_batch_m2(int tiny, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
+int launch_batch_m2(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
                     unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
 
 }  // namespace bdx

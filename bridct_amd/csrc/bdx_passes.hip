@@ -19,7 +19,7 @@ template <int H> constexpr int col_threads()
     return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
 }
 
-template <int W, int OP, bool G>
+template <int W, int OP, bool G, bool NT>
 __global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, const size_t *__restrict__ offs,
                                                     int h, const float *__restrict__ tab)
 {
@@ -28,13 +28,13 @@ __global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, 
     const size_t row0 = (size_t)blockIdx.x * R;
     if constexpr (G) {
         const size_t a = row0 / (unsigned)h, base = offs[a] + (row0 - a * h) * W;
-        row_unit<W, OP, kThreads>(in + base, out + base, 0, R, smem, tab, false);
+        row_unit<W, OP, kThreads, NT>(in + base, out + base, 0, R, smem, tab, false);
     } else {
-        row_unit<W, OP, kThreads>(in, out, row0, R, smem, tab, false);
+        row_unit<W, OP, kThreads, NT>(in, out, row0, R, smem, tab, false);
     }
 }
 
-template <int H, int MODE, bool G>
+template <int H, int MODE, bool G, bool NT>
 __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, float *out, int w,
                                                             const size_t *__restrict__ offs,
                                                             const float *__restrict__ tab)
@@ -50,10 +50,10 @@ __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, floa
     const unsigned bid = nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
     const size_t a = bid / strips;
     const int c0 = (int)(bid - a * strips) * L;
-    col_unit<H, MODE, TH>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
+    col_unit<H, MODE, TH, NT>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
 }
 
-template <int W, bool G>
+template <int W, bool G, bool NT>
 int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
             int h)
 {
@@ -62,10 +62,10 @@ int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab,
         return -1;
     const dim3 grid((unsigned)(nrows / R)), block(kThreads);
     switch (op) {
-    case FA: hipLaunchKernelGGL((k_rows<W, FA, G>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case FA2: hipLaunchKernelGGL((k_rows<W, FA2, G>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case IA: hipLaunchKernelGGL((k_rows<W, IA, G>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case I0: hipLaunchKernelGGL((k_rows<W, I0, G>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case FA: hipLaunchKernelGGL((k_rows<W, FA, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case FA2: hipLaunchKernelGGL((k_rows<W, FA2, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case IA: hipLaunchKernelGGL((k_rows<W, IA, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case I0: hipLaunchKernelGGL((k_rows<W, I0, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -73,13 +73,16 @@ int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab,
 
 template <int W>
 int rows_w(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
-           int h)
+           int h, bool nt)
 {
-    return offs ? rows_wg<W, true>(in, out, nrows, op, tab, s, offs, h)
-                : rows_wg<W, false>(in, out, nrows, op, tab, s, nullptr, 0);
+    if (offs)
+        return nt ? rows_wg<W, true, true>(in, out, nrows, op, tab, s, offs, h)
+                  : rows_wg<W, true, false>(in, out, nrows, op, tab, s, offs, h);
+    return nt ? rows_wg<W, false, true>(in, out, nrows, op, tab, s, nullptr, 0)
+              : rows_wg<W, false, false>(in, out, nrows, op, tab, s, nullptr, 0);
 }
 
-template <int H, bool G>
+template <int H, bool G, bool NT>
 int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
             const size_t *offs)
 {
@@ -88,9 +91,9 @@ int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const fl
         return -1;
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(col_threads<H>());
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_cols<H, 0, G>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 1: hipLaunchKernelGGL((k_cols<H, 1, G>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 2: hipLaunchKernelGGL((k_cols<H, 2, G>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 0: hipLaunchKernelGGL((k_cols<H, 0, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 1: hipLaunchKernelGGL((k_cols<H, 1, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 2: hipLaunchKernelGGL((k_cols<H, 2, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -98,36 +101,39 @@ int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const fl
 
 template <int H>
 int cols_h(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
-           const size_t *offs)
+           const size_t *offs, bool nt)
 {
-    return offs ? cols_hg<H, true>(w, in, out, batch, mode, tab, s, offs)
-                : cols_hg<H, false>(w, in, out, batch, mode, tab, s, nullptr);
+    if (offs)
+        return nt ? cols_hg<H, true, true>(w, in, out, batch, mode, tab, s, offs)
+                  : cols_hg<H, true, false>(w, in, out, batch, mode, tab, s, offs);
+    return nt ? cols_hg<H, false, true>(w, in, out, batch, mode, tab, s, nullptr)
+              : cols_hg<H, false, false>(w, in, out, batch, mode, tab, s, nullptr);
 }
 
 }  // namespace
 
 int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
-                const size_t *offs, int h)
+                const size_t *offs, int h, bool nt)
 {
     switch (w) {
-    case 64: return rows_w<64>(in, out, nrows, op, tab, s, offs, h);
-    case 128: return rows_w<128>(in, out, nrows, op, tab, s, offs, h);
-    case 256: return rows_w<256>(in, out, nrows, op, tab, s, offs, h);
-    case 512: return rows_w<512>(in, out, nrows, op, tab, s, offs, h);
-    case 1024: return rows_w<1024>(in, out, nrows, op, tab, s, offs, h);
+    case 64: return rows_w<64>(in, out, nrows, op, tab, s, offs, h, nt);
+    case 128: return rows_w<128>(in, out, nrows, op, tab, s, offs, h, nt);
+    case 256: return rows_w<256>(in, out, nrows, op, tab, s, offs, h, nt);
+    case 512: return rows_w<512>(in, out, nrows, op, tab, s, offs, h, nt);
+    case 1024: return rows_w<1024>(in, out, nrows, op, tab, s, offs, h, nt);
     default: return -1;
     }
 }
 
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
-                hipStream_t s, const size_t *offs)
+                hipStream_t s, const size_t *offs, bool nt)
 {
     switch (h) {
-    case 64: return cols_h<64>(w, in, out, batch, mode, tab, s, offs);
-    case 128: return cols_h<128>(w, in, out, batch, mode, tab, s, offs);
-    case 256: return cols_h<256>(w, in, out, batch, mode, tab, s, offs);
-    case 512: return cols_h<512>(w, in, out, batch, mode, tab, s, offs);
-    case 1024: return cols_h<1024>(w, in, out, batch, mode, tab, s, offs);
+    case 64: return cols_h<64>(w, in, out, batch, mode, tab, s, offs, nt);
+    case 128: return cols_h<128>(w, in, out, batch, mode, tab, s, offs, nt);
+    case 256: return cols_h<256>(w, in, out, batch, mode, tab, s, offs, nt);
+    case 512: return cols_h<512>(w, in, out, batch, mode, tab, s, offs, nt);
+    case 1024: return cols_h<1024>(w, in, out, batch, mode, tab, s, offs, nt);
     default: return -1;
     }
 }

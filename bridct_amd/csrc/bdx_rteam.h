@@ -60,7 +60,7 @@ template <int T> __device__ __forceinline__ int xswz(int k1, int j)
     return k1 * T + (j ^ (k1 & (T - 1)));
 }
 
-template <int n, int OP>
+template <int n, int OP, bool NT = false>
 __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, float *xl, const float *lt)
 {
     using R = RowTab<n>;
@@ -70,7 +70,7 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
         float4 v[C];
 #pragma unroll
         for (int c = 0; c < C; c++)
-            v[c] = *reinterpret_cast<const float4 *>(src + 4 * t + 4 * T * c);
+            v[c] = ld4<NT>(src + 4 * t + 4 * T * c);
 #pragma unroll
         for (int c = 0; c < C; c++) {
             re[c] = v[c].x;
@@ -175,14 +175,14 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
 #pragma unroll
     for (int c = 0; c < n / (4 * T); c++) {
         const int e = 4 * t + 4 * T * c;
-        *reinterpret_cast<float4 *>(dst + e) = make_float4(xl[e], xl[e + 1], xl[e + 2], xl[e + 3]);
+        st4<NT>(dst + e, make_float4(xl[e], xl[e + 1], xl[e + 2], xl[e + 3]));
     }
     team_sync();
 }
 
 // Inverse: inputs at the lane-owned output indices of the forward, outputs as float4
 // (the mirror of the forward load).
-template <int n, int OP>
+template <int n, int OP, bool NT = false>
 __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, float *xl, const float *lt)
 {
     using R = RowTab<n>;
@@ -192,8 +192,7 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
     // lines would be fetched twice once the rows in flight outgrow the L2).
 #pragma unroll
     for (int c = 0; c < n / (4 * T); c++)
-        *reinterpret_cast<float4 *>(xl + 4 * t + 4 * T * c) =
-            *reinterpret_cast<const float4 *>(src + 4 * t + 4 * T * c);
+        *reinterpret_cast<float4 *>(xl + 4 * t + 4 * T * c) = ld4<NT>(src + 4 * t + 4 * T * c);
     team_sync();
     float dr[Q][2][T], di[Q][2][T];
 #pragma unroll
@@ -299,7 +298,7 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
     for (int c = 0; c < C; c++) {
         const float4 v = make_float4(re[c], team_mirror<T>(im[N1 - 1 - c]), im[c],
                                      team_mirror<T>(re[N1 - 1 - c]));
-        *reinterpret_cast<float4 *>(dst + 4 * t + 4 * T * c) = v;
+        st4<NT>(dst + 4 * t + 4 * T * c, v);
     }
 }
 

@@ -149,22 +149,22 @@ int bdx_device_cus(void)
 
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
                      unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, int tiny, void *stream)
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, int flags, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     // a fill kernel, not a memset node, so that the sequence can be captured in a graph; the
     // tiny kernel uses no counters (the caller zeroed them once, for the error flag)
-    if (!tiny)
+    if (!(flags & BDX_LAUNCH_TINY))
         hipLaunchKernelGGL(k_zero, dim3((ncounters + 255) / 256), dim3(256), 0, s, d_counters, ncounters);
     if (hipGetLastError() != hipSuccess || nitems == 0)
         return nitems == 0 ? 0 : -1;
     switch (mode) {
     case BDX_FORWARD:
-        return bdx::launch_batch_m0(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m0(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     case BDX_INVERSE:
-        return bdx::launch_batch_m1(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m1(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     case BDX_ROUNDTRIP:
-        return bdx::launch_batch_m2(tiny, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m2(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
     default:
         return -1;
     }
@@ -212,10 +212,11 @@ int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t bat
     hipStream_t s = (hipStream_t)stream;
     if (bdx::small_route(h, w))
         return pass == 0 ? small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s) : 0;
+    const bool nt = 2 * batch * (size_t)h * w * sizeof(float) > BDX_STREAM_MIN_BYTES;
     if (pass == 0)
         return bdx::launch_rows(w, d_in, d_out, batch * (size_t)h, mode == BDX_FORWARD ? bdx::FA : bdx::IA,
-                                d_tables, s);
-    return bdx::launch_cols(h, w, d_in, d_out, batch, mode, d_tables, s);
+                                d_tables, s, nullptr, 0, nt);
+    return bdx::launch_cols(h, w, d_in, d_out, batch, mode, d_tables, s, nullptr, nt);
 }
 
 static int transform(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,
@@ -229,10 +230,11 @@ static int transform(const float *d_in, float *d_out, int h, int w, size_t batch
         return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s, offs);
     const size_t nrows = batch * (size_t)h;
     const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;
-    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s, offs, h) ||
-        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s, offs))
+    const bool nt = 2 * nrows * (size_t)w * sizeof(float) > BDX_STREAM_MIN_BYTES;
+    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s, offs, h, nt) ||
+        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s, offs, nt))
         return -1;
-    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s, offs, h))
+    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s, offs, h, nt))
         return -1;
     return 0;
 }

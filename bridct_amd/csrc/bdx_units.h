@@ -24,7 +24,7 @@ template <int H, int W, int THREADS> constexpr int small_per_unit(int lds_floats
 
 // `count` whole arrays; row r of the sequence of their rows (array r / H, row r % H) is read
 // at src(r) and written at dst(r).
-template <int H, int W, int MODE, int THREADS, typename Src, typename Dst>
+template <int H, int W, int MODE, int THREADS, bool NT = false, typename Src, typename Dst>
 __device__ __forceinline__ void small_unit_rows(Src src, Dst dst, int count, float *tile,
                                                 const float *__restrict__ tab)
 {
@@ -34,7 +34,7 @@ __device__ __forceinline__ void small_unit_rows(Src src, Dst dst, int count, flo
         const int a = L / W, c = L - a * W;
         return a * H * P + c;
     };
-    tile_load_rows<W, P, THREADS>(tile, src, count * H);
+    tile_load_rows<W, P, THREADS, NT>(tile, src, count * H);
     __syncthreads();
     if constexpr (MODE == 0) {
         run_lines<W, 1, FA, THREADS>(tile, count * H, row_base, tab);
@@ -54,16 +54,16 @@ __device__ __forceinline__ void small_unit_rows(Src src, Dst dst, int count, flo
         run_lines<W, 1, I0, THREADS>(tile, count * H, row_base, tab);
     }
     __syncthreads();
-    tile_store_rows<W, P, THREADS>(tile, dst, count * H);
+    tile_store_rows<W, P, THREADS, NT>(tile, dst, count * H);
 }
 
 // `count` contiguous whole arrays starting at element `off`.
-template <int H, int W, int MODE, int THREADS>
+template <int H, int W, int MODE, int THREADS, bool NT = false>
 __device__ __forceinline__ void small_unit(const float *in, float *out, size_t off, int count, float *tile,
                                            const float *__restrict__ tab)
 {
-    small_unit_rows<H, W, MODE, THREADS>([=](int r) { return in + off + (size_t)r * W; },
-                                         [=](int r) { return out + off + (size_t)r * W; }, count, tile, tab);
+    small_unit_rows<H, W, MODE, THREADS, NT>([=](int r) { return in + off + (size_t)r * W; },
+                                             [=](int r) { return out + off + (size_t)r * W; }, count, tile, tab);
 }
 
 // ---- row pass ------------------------------------------------------------------------
@@ -82,7 +82,7 @@ template <int W, int OP, int THREADS> constexpr int row_unit_lds()
 // Rows [row0, row0 + nrows) of the row sequence (all rows of all arrays of the same width
 // are one sequence), nrows <= row_unit_rows.  With `tables_ready` the LDS constant table
 // of width W is already in smem.
-template <int W, int OP, int THREADS>
+template <int W, int OP, int THREADS, bool NT = false>
 __device__ __forceinline__ void row_unit(const float *in, float *out, size_t row0, int nrows, float *smem,
                                          const float *__restrict__ tab, bool tables_ready)
 {
@@ -99,17 +99,17 @@ __device__ __forceinline__ void row_unit(const float *in, float *out, size_t row
         const size_t row = row0 + team;
         if (team < nrows) {   // a team works alone on its row: no block barrier follows
             if constexpr (OP == FA || OP == FA2)
-                rteam_fwd<W, OP>(in + row * W, out + row * W, t, xl + team * X, lt);
+                rteam_fwd<W, OP, NT>(in + row * W, out + row * W, t, xl + team * X, lt);
             else
-                rteam_inv<W, OP>(in + row * W, out + row * W, t, xl + team * X, lt);
+                rteam_inv<W, OP, NT>(in + row * W, out + row * W, t, xl + team * X, lt);
         }
     } else {
         constexpr int P = W + 1;
-        tile_load<W, P, THREADS>(smem, in + row0 * W, W, nrows);
+        tile_load<W, P, THREADS, NT>(smem, in + row0 * W, W, nrows);
         __syncthreads();
         run_lines<W, 1, OP, THREADS>(smem, nrows, [](int L) { return L * P; }, tab);
         __syncthreads();
-        tile_store<W, P, THREADS>(smem, out + row0 * W, W, nrows);
+        tile_store<W, P, THREADS, NT>(smem, out + row0 * W, W, nrows);
     }
 }
 
@@ -134,7 +134,7 @@ template <int H, int THREADS> constexpr int col_unit_lds()
 // Columns [c0, c0 + ncols) of the h x w array at `base` (in and out may alias),
 // ncols <= col_unit_cols; lanes past the strip compute on zeros and store nothing.
 // MODE 0 forward * alpha, 1 alpha * inverse, 2 forward * alpha^2 then inverse.
-template <int H, int MODE, int THREADS>
+template <int H, int MODE, int THREADS, bool NT = false>
 __device__ __forceinline__ void col_unit(const float *in, float *out, int w, size_t base, int c0, int ncols,
                                          float *smem, const float *__restrict__ tab, bool tables_ready)
 {
@@ -152,10 +152,10 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         const float *src = in + base + c0;
         float *dst = out + base + c0;
         const bool live = line < ncols;
-        const auto ld = [&](int i) { return live ? src[(unsigned)(i * w) + line] : 0.0f; };
+        const auto ld = [&](int i) { return live ? ld1<NT>(src + (unsigned)(i * w) + line) : 0.0f; };
         const auto st = [&](int i, float v) {
             if (live)
-                dst[(unsigned)(i * w) + line] = v;
+                st1<NT>(dst + (unsigned)(i * w) + line, v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
         const CConst<H, M> cc{tab + BDX_TABLE_BASE(H), lt};
@@ -192,7 +192,7 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         float x[H];
 #pragma unroll
         for (int i = 0; i < H; i++)
-            x[i] = in[off + (size_t)i * w] * (MODE == 1 ? al[i] : 1.0f);
+            x[i] = ld1<NT>(in + off + (size_t)i * w) * (MODE == 1 ? al[i] : 1.0f);
         if constexpr (MODE == 1) {
             Sj<H>::inv(x);
         } else {
@@ -205,7 +205,7 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         }
 #pragma unroll
         for (int i = 0; i < H; i++)
-            out[off + (size_t)i * w] = x[i];
+            st1<NT>(out + off + (size_t)i * w, x[i]);
     }
 }
 

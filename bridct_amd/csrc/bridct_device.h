@@ -43,6 +43,11 @@ extern "C" {
 #define BDX_ROW_UNIT(w) (BDX_ROW_BAND(w) * ((w) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_COL_UNIT(h) (BDX_COL_STRIP(h) * ((h) >= 512 ? BDX_ITEM_BANDS : 1))
 #define BDX_CEIL(a, b) (((a) + (b) - 1) / (b))
+/* Calls whose input plus output exceed this many bytes (the 256 MB Infinity Cache of an
+ * MI355X) stream their array data with non-temporal loads and stores: the two-pass
+ * kernels and the tiny batch kernel. */
+#define BDX_STREAM_MIN_BYTES ((size_t)256 << 20)
+
 /* Whole-array route of the plan path: an h x w array with h (w + 1) <= BDX_SMALL_LDS_FLOATS
  * is transformed by one workgroup in LDS. */
 #define BDX_SMALL_LDS_FLOATS 36864
@@ -82,9 +87,13 @@ typedef struct {
 /* Launches the persistent batch kernel (after a counter reset) on `stream`:
  * `blocks` resident workgroups pull `nitems` items.  spin_limit bounds each dependency
  * wait (a wait that runs out sets the error flag instead of hanging). */
+/* flags of bdx_batch_launch: the tiny kernel instead of the persistent one, and streaming
+ * (non-temporal) array accesses in it */
+#define BDX_LAUNCH_TINY 1
+#define BDX_LAUNCH_STREAM 2
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
                      unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, int tiny, void *stream);
+                     const float *d_tables, unsigned blocks, unsigned spin_limit, int flags, void *stream);
 int bdx_device_cus(void);
 
 /* Host-side schedule of a batch (no device needed): runs of consecutive equal shapes,
