@@ -207,11 +207,17 @@ def committed_traffic(kernel):
     return None
 
 
-def leg_copy(hip):
+def leg_copy(hip, lib):
+    """256 MiB device copies: hipMemcpy, and the library's streaming float4 copy kernel (the
+    practical ceiling of one read and one write per element)."""
     n = 256 << 20
     a, b = capi.DeviceBuffer(n), capi.DeviceBuffer(n)
     ms = hip.time_ms(None, lambda: hip.copy_d2d(b.ptr, a.ptr, n), 10)
-    return {'bytes': n, 'ms': round(ms, 4), 'gbs': round(2 * n / (ms * 1e-3) / 1e9, 1)}
+    lib.bdx_copy_stream.restype = ctypes.c_int
+    lib.bdx_copy_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    ks = hip.time_ms(None, lambda: lib.bdx_copy_stream(a.ptr, b.ptr, n // 4, None), 10)
+    return {'bytes': n, 'ms': round(ms, 4), 'gbs': round(2 * n / (ms * 1e-3) / 1e9, 1),
+            'stream_kernel_ms': round(ks, 4), 'stream_kernel_gbs': round(2 * n / (ks * 1e-3) / 1e9, 1)}
 
 
 def leg_blocks(hip, lib):
@@ -557,7 +563,7 @@ def main():
         'roundtrip_max_rel_err_1024': rt_err,
     }
     if 'copy' in legs:
-        line['copy_reference'] = leg_copy(hip)
+        line['copy_reference'] = leg_copy(hip, lib)
     if 'c1' in legs:
         line['plain_256x256'] = leg_c1(hip, lib)
     if 'dense' in legs:

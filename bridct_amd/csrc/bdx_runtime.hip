@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include "bdx_kernels.h"
+#include "bdx_lines.h"
 #include "bridct_device.h"
 
 namespace {
@@ -29,6 +30,14 @@ int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int 
     case 1024: return launch_small_h1024(w, in, out, batch, pack, mode, tab, s, offs);
     default: return -1;
     }
+}
+
+// Device copy with streaming float4 accesses: the practical HBM ceiling that the transform
+// kernels are compared with (a read and a write per element, like one pass).
+__global__ __launch_bounds__(256) void k_copy_stream(const float *__restrict__ a, float *__restrict__ b, size_t n4)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256)
+        bdx::st4<true>(b + 4 * i, bdx::ld4<true>(a + 4 * i));
 }
 
 __global__ void k_zero(unsigned *p, unsigned n)
@@ -97,6 +106,16 @@ int bdx_event_record(void *event, void *stream)
 int bdx_event_sync(void *event)
 {
     return hipEventSynchronize((hipEvent_t)event) == hipSuccess ? 0 : -1;
+}
+
+int bdx_copy_stream(const float *d_src, float *d_dst, size_t floats, void *stream)
+{
+    if (floats % 4 || ((uintptr_t)d_src | (uintptr_t)d_dst) % 16)
+        return -1;
+    const size_t n4 = floats / 4, blocks = (n4 + 255) / 256;
+    hipLaunchKernelGGL(k_copy_stream, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                       (hipStream_t)stream, d_src, d_dst, n4);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int bdx_stream_wait_event(void *stream, void *event)

@@ -142,6 +142,9 @@ int bdx_stream_sync(void *stream);
 int bdx_copy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int bdx_copy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int bdx_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* Streaming float4 device copy (floats a multiple of 4, 16-byte aligned): the measured
+ * HBM ceiling for one read and one write per element. */
+int bdx_copy_stream(const float *d_src, float *d_dst, size_t floats, void *stream);
 
 /* Orthonormal 2D transform of `batch` contiguous row-major h x w float32 arrays.
  * d_in and d_out must be 16-byte aligned and either identical or disjoint.  pack: arrays
