@@ -186,11 +186,11 @@ def cpu_baseline(inputs):
 
 
 KERNELS_1024 = (
-    # (label, rocprof kernel name, mode, pass)
-    ('row pass, forward', 'k_rows<1024, 1>', capi.FORWARD, 0),
-    ('column pass, forward', 'k_cols<1024, 0>', capi.FORWARD, 1),
-    ('row pass, inverse', 'k_rows<1024, 3>', capi.INVERSE, 0),
-    ('column pass, inverse', 'k_cols<1024, 1>', capi.INVERSE, 1),
+    # (label, rocprof kernel name: <length, op or mode, gather form, streaming>, mode, pass)
+    ('row pass, forward', 'k_rows<1024, 1, false, true>', capi.FORWARD, 0),
+    ('column pass, forward', 'k_cols<1024, 0, false, true>', capi.FORWARD, 1),
+    ('row pass, inverse', 'k_rows<1024, 3, false, true>', capi.INVERSE, 0),
+    ('column pass, inverse', 'k_cols<1024, 1, false, true>', capi.INVERSE, 1),
 )
 
 
