@@ -7,9 +7,9 @@ Gaussian blobs, amplitude renormalized into [-1, 1]) plus N(0, 0.01^2) noise.  O
 is the orthonormal forward DCT-II and then the inverse DCT-III of every batch, with inputs
 already resident in HBM.  Prints one JSON line: whole-job Gpixel/s of forward + inverse,
 per-shape forward / inverse rates, the HBM roofline of the dominant kernel (each kernel of
-the 64 x 1024^2 transforms timed alone with HIP events on the plan's stream), a device
-copy as the practical bandwidth reference, and the reference CPU path timed on this host's
-cores.  Secondary legs (not part of `value`): the dense split-MFMA route vs the
+the 64 x 1024^2 transforms timed in sequence, row pass then column pass, with HIP events on
+the plan's stream), device copies as the practical bandwidth reference, and the reference
+CPU path timed on this host's cores.  Secondary legs (not part of `value`): the dense split-MFMA route vs the
 butterflies at 64^2, 256^2, 1024^2 (batch 128), 1,048,576 independent 8x8 blocks in one
 persistent launch vs per-block launches, and a ragged batch of 10,000 arrays of random
 supported shapes (seed 7) in one persistent launch vs one plan launch per array, and the
@@ -533,15 +533,28 @@ def main():
     runs[0][3][2].download(back)
     rt_err = float(np.abs(back - x).max() / np.abs(x).max())
 
-    # each kernel of the 1024^2 transforms alone: 8 B per pixel (one read, one write)
+    # each kernel of the 1024^2 transforms as it runs in the transform (row pass, then the
+    # column pass on its output), with HIP events between the launches on the plan stream:
+    # 8 B per pixel per kernel (one read, one write)
     h, w, plan, (a, b, c) = runs[0]
     px = 1024 * 1024 * BATCH
     kernels = {}
-    for label, kname, mode, pss in KERNELS_1024:
-        src = a if pss == 0 else b
-        ms = hip.time_ms(plan.stream, lambda: plan.apply_device_pass(mode, pss, src.ptr, b.ptr, BATCH), 5)
-        kernels[kname] = {'what': label, 'ms': round(ms, 4),
-                          'gbs': round(8.0 * px / (ms * 1e-3) / 1e9, 1)}
+    reps = 6
+    for mode in (capi.FORWARD, capi.INVERSE):
+        ev = [hip.event() for _ in range(2 * reps + 1)]
+        hip.record(ev[0], plan.stream)
+        for r in range(reps):   # back to back, so that the queue never runs dry between events
+            assert plan.apply_device_pass(mode, 0, a.ptr, b.ptr, BATCH) == 0
+            hip.record(ev[2 * r + 1], plan.stream)
+            assert plan.apply_device_pass(mode, 1, b.ptr, b.ptr, BATCH) == 0
+            hip.record(ev[2 * r + 2], plan.stream)
+        times = [hip.elapsed_ms(ev[i], ev[i + 1]) for i in range(2 * reps)]
+        for e in ev:
+            hip.lib.hipEventDestroy(e)
+        for pss in (0, 1):
+            ms = sum(times[2 + pss::2]) / (reps - 1)   # the first repetition is a warm-up
+            label, kname = next((l, k) for l, k, m, p in KERNELS_1024 if m == mode and p == pss)
+            kernels[kname] = {'what': label, 'ms': round(ms, 4), 'gbs': round(8.0 * px / (ms * 1e-3) / 1e9, 1)}
     dom_name = max(kernels, key=lambda k: kernels[k]['ms'])
     dom = kernels[dom_name]
     achieved = 8.0 * px / (dom['ms'] * 1e-3) / 1e9
