@@ -34,12 +34,12 @@ __global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, 
     }
 }
 
-template <int H, int MODE, bool G, bool NT>
-__global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, float *out, int w,
-                                                            const size_t *__restrict__ offs,
-                                                            const float *__restrict__ tab)
+// Strips of TH / 64 * (64 / M) columns per workgroup of TH lanes.
+template <int H, int MODE, bool G, bool NT, int TH>
+__device__ __forceinline__ void cols_body(const float *in, float *out, int w, const size_t *__restrict__ offs,
+                                          const float *__restrict__ tab)
 {
-    constexpr int TH = col_threads<H>(), L = col_unit_cols<H, TH>();
+    constexpr int L = col_unit_cols<H, TH>();
     __shared__ float smem[col_unit_lds<H, TH>()];
     const int strips = w / L;
     // Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD b % kXcds);
@@ -52,6 +52,30 @@ __global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, floa
     const int c0 = (int)(bid - a * strips) * L;
     col_unit<H, MODE, TH, NT>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
 }
+
+template <int H, int MODE, bool G, bool NT>
+__global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, float *out, int w,
+                                                            const size_t *__restrict__ offs,
+                                                            const float *__restrict__ tab)
+{
+    cols_body<H, MODE, G, NT, col_threads<H>()>(in, out, w, offs, tab);
+}
+
+// The same column pass with narrower strips (TH = col_threads / 2 or / 4: 32 or 16
+// columns, several team members per wave), for grids that would otherwise leave CUs idle:
+// e.g. 64 arrays of 512 x 128 are 128 workgroups of 64 columns.  Team members and
+// arithmetic are those of k_cols; only the strip width differs, so the results are equal
+// bitwise (the LDS copy of the team twiddles holds the same floats as the global table).
+template <int H, int MODE, bool G, int TH>
+__global__ __launch_bounds__(TH) void k_cols_narrow(const float *in, float *out, int w,
+                                                     const size_t *__restrict__ offs,
+                                                     const float *__restrict__ tab)
+{
+    cols_body<H, MODE, G, false, TH>(in, out, w, offs, tab);
+}
+
+// Workgroups below which a column pass takes narrower strips: two per CU.
+constexpr size_t kColsMinGrid = 512;
 
 template <int W, bool G, bool NT>
 int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
@@ -82,14 +106,44 @@ int rows_w(const float *in, float *out, size_t nrows, int op, const float *tab, 
               : rows_wg<W, false, false>(in, out, nrows, op, tab, s, nullptr, 0);
 }
 
+template <int H, bool G, int TH>
+int cols_narrow(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
+                const size_t *offs)
+{
+    constexpr int L = col_unit_cols<H, TH>();
+    const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_cols_narrow<H, 0, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 1: hipLaunchKernelGGL((k_cols_narrow<H, 1, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 2: hipLaunchKernelGGL((k_cols_narrow<H, 2, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <int H, bool G, bool NT>
 int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
             const size_t *offs)
 {
-    constexpr int L = col_unit_cols<H, col_threads<H>()>();
+    constexpr int TH = col_threads<H>(), L = col_unit_cols<H, TH>();
     if (w % L)
         return -1;
-    const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(col_threads<H>());
+    // Small grids (never streaming ones: those exceed the 256 MB cache) take strips of half
+    // or a quarter of the width, down to 16 columns, until there are two workgroups per CU.
+    if constexpr (!NT && H >= 128) {
+        constexpr int L2 = col_unit_cols<H, TH / 2>(), L4 = col_unit_cols<H, TH / 4>();
+        static_assert(L2 == L / 2 && L4 == L / 4, "narrow strips split the team's lines");
+        if (batch * (size_t)(w / L) < kColsMinGrid) {
+            if constexpr (L4 >= 16) {
+                if (batch * (size_t)(w / L2) >= kColsMinGrid)
+                    return cols_narrow<H, G, TH / 2>(w, in, out, batch, mode, tab, s, offs);
+                return cols_narrow<H, G, TH / 4>(w, in, out, batch, mode, tab, s, offs);
+            } else if constexpr (L2 >= 16) {
+                return cols_narrow<H, G, TH / 2>(w, in, out, batch, mode, tab, s, offs);
+            }
+        }
+    }
+    const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
     switch (mode) {
     case 0: hipLaunchKernelGGL((k_cols<H, 0, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
     case 1: hipLaunchKernelGGL((k_cols<H, 1, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;

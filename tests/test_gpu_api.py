@@ -233,3 +233,35 @@ def test_passes_compose_to_apply_device(h, w):
         assert plan.apply_device_pass(capi.FORWARD, 1, a.ptr, b.ptr, batch) == capi.INVALID_ARGUMENT
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize('h,w', [(128, 512), (256, 256), (512, 128), (1024, 64), (128, 1024), (512, 512),
+                                 (1024, 1024)])
+def test_narrow_column_strips_bitwise(h, w):
+    """Column passes with too few workgroups for the GPU take strips of a half or a quarter
+    of the width.  The same arrays transformed in batches that pick the wide, the half and
+    the quarter strips give the same bits, in every mode."""
+    rng = np.random.default_rng(h * 7 + w)
+    full = 32 * 1024 // w    # 512 workgroups of 64 columns: the wide strips
+    x = rng.uniform(-1, 1, (full, h, w)).astype(np.float32)
+    plan = capi.Plan(h, w)
+    a, b = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    try:
+        a.upload(x)
+        for mode in (capi.FORWARD, capi.INVERSE, capi.ROUNDTRIP):
+            print('narrow strips %dx%d mode %d' % (h, w, mode), flush=True)
+            ref = None
+            for batch in (full, max(1, full // 2), max(1, full // 8), 1):
+                assert plan.apply_device(mode, a.ptr, b.ptr, batch) == 0
+                assert plan.synchronize() == 0
+                y = np.empty((full, h, w), np.float32)
+                b.download(y)
+                if ref is None:
+                    ref = y.copy()
+                    assert np.isfinite(ref).all()
+                else:
+                    assert np.array_equal(y[:batch], ref[:batch]), batch
+    finally:
+        a.free()
+        b.free()
+        plan.close()
