@@ -473,8 +473,10 @@ def leg_ragged(hip, lib):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=10)
-    ap.add_argument('--warmup', type=int, default=3)
+    # a four-shape step takes about 0.5 ms; the GPU reaches its steady clock after about 40
+    # steps, so the defaults warm up for 50 steps (25 ms) and time 50
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--legs', default='c2,copy,c1,dense,blocks,ragged,codec',
                     help='comma list of legs: c2 (always run), copy, c1, dense, blocks, ragged, codec')

@@ -57,7 +57,8 @@ template <int n> constexpr int team_size() { return BDX_TEAM(n); }
 // through the caches).  The kernels take NT for calls whose input and output exceed the
 // 256 MB Infinity Cache (BDX_STREAM_MIN_BYTES): a float4 copy kernel reaches 6.7 TB/s with
 // the hint against 6.2 TB/s without it on MI355X, but batches that fit the cache ran up to
-// 13% slower with it.
+// 13% slower with it.  The row pass of a two-pass transform streams its loads only (its
+// output is read again by the column pass).
 typedef float f4v __attribute__((ext_vector_type(4)));
 template <bool NT> __device__ __forceinline__ float4 ld4(const float *p)
 {

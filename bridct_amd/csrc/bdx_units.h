@@ -81,7 +81,8 @@ template <int W, int OP, int THREADS> constexpr int row_unit_lds()
 
 // Rows [row0, row0 + nrows) of the row sequence (all rows of all arrays of the same width
 // are one sequence), nrows <= row_unit_rows.  With `tables_ready` the LDS constant table
-// of width W is already in smem.
+// of width W is already in smem.  NT streams the loads only: the output stays cacheable
+// for the column pass that reads it next.
 template <int W, int OP, int THREADS, bool NT = false>
 __device__ __forceinline__ void row_unit(const float *in, float *out, size_t row0, int nrows, float *smem,
                                          const float *__restrict__ tab, bool tables_ready)
@@ -109,7 +110,7 @@ __device__ __forceinline__ void row_unit(const float *in, float *out, size_t row
         __syncthreads();
         run_lines<W, 1, OP, THREADS>(smem, nrows, [](int L) { return L * P; }, tab);
         __syncthreads();
-        tile_store<W, P, THREADS, NT>(smem, out + row0 * W, W, nrows);
+        tile_store<W, P, THREADS, false>(smem, out + row0 * W, W, nrows);   // read again by the column pass
     }
 }
 
