@@ -473,8 +473,8 @@ def leg_ragged(hip, lib):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
-    # a four-shape step takes about 0.5 ms; the GPU reaches its steady clock after about 40
-    # steps, so the defaults warm up for 50 steps (25 ms) and time 50
+    # a four-shape step takes about 0.5 ms and reaches its steady time after about 40 steps,
+    # so the defaults warm up for 50 steps (25 ms) and time 50
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
