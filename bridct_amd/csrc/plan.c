@@ -15,6 +15,7 @@
 
 #include "bridct_device.h"
 #include "generated/sj_factors.h"
+#include "copy_pool.h"
 #include "host_internal.h"
 
 #include <math.h>
@@ -29,11 +30,13 @@ static const double PI = 3.14159265358979323846;
 #define STAGE_FLOATS ((size_t)1 << 20)
 
 /* Host calls of at least this many floats go through the pinned pipeline: two chunk slots,
- * each with a pinned input and output buffer, a device input and output buffer and an
- * event.  The host copies chunk k into pinned memory while the GPU copies in, transforms
- * and copies out chunk k - 1, then copies finished chunks out of pinned memory.  Pageable
- * copies run at 31 GB/s (in) and 18 GB/s (out) here; pinned ones at 57 GB/s each way, so
- * the host memcpy becomes the bound. */
+ * each with a pinned input and output buffer, a device input and output buffer and two
+ * events.  The host copies chunk k - 2 out of pinned memory and chunk k into it while the
+ * GPU copies in, transforms and copies out chunk k - 1; the copies in run on a stream of
+ * their own, so that they overlap the copies out on the other DMA engine.  Pageable
+ * copies run at 31 GB/s (in) and 18 GB/s (out) here; pinned ones at 57 GB/s each way and
+ * one thread's memcpy at 32 GB/s, so the host copies are shared by the threads of a
+ * copy_pool (copy_pool.h). */
 #define PIPELINE_MIN_FLOATS ((size_t)1 << 18)
 
 struct bridct_plan {
@@ -45,7 +48,10 @@ struct bridct_plan {
     float *d_in, *d_out;  /* chunk * elements each */
     /* pinned pipeline for large host calls, allocated on first use */
     float *pin_in[2], *pin_out[2], *d_in2, *d_out2;
-    void *ev[2];
+    void *ev[2];          /* chunk of the slot copied out (compute stream) */
+    void *ev_in[2];       /* chunk of the slot copied in (copy stream) */
+    void *copy_stream;    /* hipStream_t of the copies in */
+    copy_pool *pool;
     int route;            /* BRIDCT_ROUTE_* */
     int pack;             /* arrays per workgroup of the whole-array kernel, 0 = automatic */
     float *d_dense;       /* dense route: Q_h', Q_h'^T, Q_w' */
@@ -130,6 +136,8 @@ void bridct_plan_destroy(bridct_plan *plan)
 {
     if (!plan)
         return;
+    if (plan->copy_stream)
+        bdx_stream_sync(plan->copy_stream);
     if (plan->stream)
         bdx_stream_sync(plan->stream);
     if (plan->d_dense)
@@ -153,7 +161,12 @@ void bridct_plan_destroy(bridct_plan *plan)
             bdx_host_free(plan->pin_out[i]);
         if (plan->ev[i])
             bdx_event_destroy(plan->ev[i]);
+        if (plan->ev_in[i])
+            bdx_event_destroy(plan->ev_in[i]);
     }
+    pool_destroy(plan->pool);
+    if (plan->copy_stream)
+        bdx_stream_destroy(plan->copy_stream);
     if (plan->stream)
         bdx_stream_destroy(plan->stream);
     free(plan);
@@ -350,79 +363,64 @@ static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_ou
 
 static int pipeline_ready(bridct_plan *plan)
 {
-    if (plan->ev[1])
+    if (plan->pool)
         return 0;
     const size_t bytes = plan->chunk * plan->elements * sizeof(float);
     for (int i = 0; i < 2; i++)
         if ((!plan->pin_in[i] && bdx_host_malloc((void **)&plan->pin_in[i], bytes)) ||
-            (!plan->pin_out[i] && bdx_host_malloc((void **)&plan->pin_out[i], bytes)))
+            (!plan->pin_out[i] && bdx_host_malloc((void **)&plan->pin_out[i], bytes)) ||
+            (!plan->ev[i] && bdx_event_create(&plan->ev[i])) ||
+            (!plan->ev_in[i] && bdx_event_create(&plan->ev_in[i])))
             return -1;
     if ((!plan->d_in2 && bdx_malloc((void **)&plan->d_in2, bytes)) ||
         (!plan->d_out2 && bdx_malloc((void **)&plan->d_out2, bytes)) ||
-        (!plan->ev[0] && bdx_event_create(&plan->ev[0])) || bdx_event_create(&plan->ev[1]))
+        (!plan->copy_stream && bdx_stream_create(&plan->copy_stream)))
         return -1;
-    return 0;
+    plan->pool = pool_create();   /* the last step: its presence marks the pipeline ready */
+    return plan->pool ? 0 : -1;
 }
 
-typedef struct {
-    void *dst;
-    const void *src;
-    size_t bytes;
-} copy_job;
-
-static void *copy_worker(void *arg)
-{
-    const copy_job *j = arg;
-    memcpy(j->dst, j->src, j->bytes);
-    return NULL;
-}
-
-/* Large host calls: chunks alternate between two slots (see PIPELINE_MIN_FLOATS).  The
- * copy of a finished chunk out of pinned memory runs on a helper thread while this thread
- * copies the next chunk in. */
+/* Large host calls: chunks alternate between two slots (see PIPELINE_MIN_FLOATS). */
 static int run_host_pipelined(bridct_plan *plan, int mode, const float *input, float *output, size_t batch)
 {
     const size_t per = plan->chunk, nchunks = (batch + per - 1) / per;
     float *d_in[2] = {plan->d_in, plan->d_in2}, *d_out[2] = {plan->d_out, plan->d_out2};
     size_t floats_of[2] = {0, 0};
     int rc = BRIDCT_OK;
-    for (size_t k = 0; k < nchunks + 2 && rc == BRIDCT_OK; k++) {
+    for (size_t k = 0; k < nchunks + 2; k++) {
         const int s = (int)(k & 1);
-        copy_job out_job = {0};
-        pthread_t helper;
-        int helping = 0;
-        if (k >= 2) {   /* chunk k - 2 finished in slot s: copy it out */
+        copy_job jobs[2];
+        int njobs = 0;
+        if (k >= 2) {   /* chunk k - 2 finished in slot s (copied in, transformed, copied out) */
             if (bdx_event_sync(plan->ev[s])) {
                 rc = BRIDCT_DEVICE_ERROR;
                 break;
             }
-            out_job.dst = output + (k - 2) * per * plan->elements;
-            out_job.src = plan->pin_out[s];
-            out_job.bytes = floats_of[s] * sizeof(float);
-            if (k < nchunks && pthread_create(&helper, NULL, copy_worker, &out_job) == 0)
-                helping = 1;
-            else
-                copy_worker(&out_job);
+            jobs[njobs++] = (copy_job){output + (k - 2) * per * plan->elements, plan->pin_out[s],
+                                       floats_of[s] * sizeof(float)};
         }
+        size_t count = 0;
         if (k < nchunks) {
-            const size_t count = batch - k * per < per ? batch - k * per : per;
-            const size_t floats = count * plan->elements;
-            floats_of[s] = floats;
-            memcpy(plan->pin_in[s], input + k * per * plan->elements, floats * sizeof(float));
-            if (helping) {   /* pin_out[s] must be read out before chunk k's copy lands in it */
-                pthread_join(helper, NULL);
-                helping = 0;
-            }
-            if (bdx_copy_h2d(d_in[s], plan->pin_in[s], floats * sizeof(float), plan->stream) ||
-                transform(plan, mode, d_in[s], d_out[s], count) ||
-                bdx_copy_d2h(plan->pin_out[s], d_out[s], floats * sizeof(float), plan->stream) ||
-                bdx_event_record(plan->ev[s], plan->stream))
-                rc = BRIDCT_DEVICE_ERROR;
+            count = batch - k * per < per ? batch - k * per : per;
+            floats_of[s] = count * plan->elements;
+            jobs[njobs++] = (copy_job){plan->pin_in[s], input + k * per * plan->elements,
+                                       floats_of[s] * sizeof(float)};
         }
-        if (helping)
-            pthread_join(helper, NULL);
+        pool_copy(plan->pool, jobs, njobs);
+        if (k < nchunks) {
+            const size_t bytes = floats_of[s] * sizeof(float);
+            if (bdx_copy_h2d(d_in[s], plan->pin_in[s], bytes, plan->copy_stream) ||
+                bdx_event_record(plan->ev_in[s], plan->copy_stream) ||
+                bdx_stream_wait_event(plan->stream, plan->ev_in[s]) ||
+                transform(plan, mode, d_in[s], d_out[s], count) ||
+                bdx_copy_d2h(plan->pin_out[s], d_out[s], bytes, plan->stream) ||
+                bdx_event_record(plan->ev[s], plan->stream)) {
+                rc = BRIDCT_DEVICE_ERROR;
+                break;
+            }
+        }
     }
-    if (bdx_stream_sync(plan->stream))
+    if (bdx_stream_sync(plan->copy_stream) || bdx_stream_sync(plan->stream))
         rc = BRIDCT_DEVICE_ERROR;
     return rc;
 }
