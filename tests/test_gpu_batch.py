@@ -198,6 +198,27 @@ def test_graph_replays_match_eager(strategy):
         b.close()
 
 
+@pytest.mark.parametrize('shapes', [[(512, 512), (8, 8)], [(1024, 1024)] * 5 + [(8, 8), (32, 64), (256, 128)]],
+                         ids=['one-partial-chunk', 'three-chunks'])
+def test_host_apply_staged(shapes):
+    """Host calls of 1 MiB or more go through pinned staging in chunks of 8 MiB: a single
+    partial chunk, and several chunks with a partial last one, equal the device path
+    bitwise and leave the input unchanged."""
+    arrays = ragged(shapes, 5)
+    x = flat(arrays)
+    x0 = x.copy()
+    b = capi.Batch(shapes)
+    try:
+        for mode in (0, 1, FTI):
+            print('staged host batch %d arrays mode %d' % (len(shapes), mode), flush=True)
+            y = np.full_like(x, np.nan)
+            assert b.apply(mode, x, y) == 0
+            assert np.array_equal(y, run_device(b, mode, x))
+            assert np.array_equal(x, x0)
+    finally:
+        b.close()
+
+
 @pytest.mark.parametrize('strategy', STRATEGIES)
 def test_host_apply_matches_device(strategy):
     shapes = [(64, 256), (256, 64), (8, 8), (1024, 32), (128, 128)]
