@@ -14,6 +14,7 @@
 #include "bridct_gpu.h"
 
 #include "bridct_device.h"
+#include "host_stage.h"
 #include "host_internal.h"
 
 #include <math.h>
@@ -27,7 +28,30 @@ struct bridct_codec8 {
     unsigned char *d_image;   /* host-call staging, grown on demand */
     short *d_coeffs;
     size_t image_cap, coeff_cap;
+    host_stage *stage;        /* pinned staging of large host calls, on first use */
 };
+
+/* Host-call copies of at least this many bytes are staged through pinned memory. */
+#define STAGE_MIN_BYTES ((size_t)1 << 20)
+#define STAGE_CHUNK_BYTES ((size_t)8 << 20)
+
+static int to_device(bridct_codec8 *c, void *d_dst, const void *src, size_t bytes)
+{
+    if (bytes >= STAGE_MIN_BYTES && !c->stage)
+        c->stage = stage_create(STAGE_CHUNK_BYTES);
+    return bytes >= STAGE_MIN_BYTES && c->stage ? stage_h2d(c->stage, d_dst, src, bytes, c->stream)
+                                                 : bdx_copy_h2d(d_dst, src, bytes, c->stream);
+}
+
+/* after the work issued so far; the host buffer holds the bytes on return (success) */
+static int to_host(bridct_codec8 *c, void *dst, const void *d_src, size_t bytes)
+{
+    if (bytes >= STAGE_MIN_BYTES && !c->stage)
+        c->stage = stage_create(STAGE_CHUNK_BYTES);
+    if (bytes >= STAGE_MIN_BYTES && c->stage)
+        return stage_d2h(c->stage, dst, d_src, bytes, c->stream);
+    return bdx_copy_d2h(dst, d_src, bytes, c->stream) || bdx_stream_sync(c->stream);
+}
 
 void bridct_codec8_destroy(bridct_codec8 *c)
 {
@@ -35,6 +59,7 @@ void bridct_codec8_destroy(bridct_codec8 *c)
         return;
     if (c->stream)
         bdx_stream_sync(c->stream);
+    stage_destroy(c->stage);
     void *bufs[] = {c->d_factors, c->d_image, c->d_coeffs};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
@@ -151,15 +176,15 @@ static int host_run(bridct_codec8 *c, int decode, const void *src, void *dst, si
         if (packed)
             for (size_t r = 0; r < height; r++)
                 memcpy(packed + r * width, img + r * stride, width);
-        if (bdx_copy_h2d(c->d_image, packed ? packed : img, image_bytes, c->stream) ||
+        if (to_device(c, c->d_image, packed ? packed : img, image_bytes) ||
             bdx_codec8(0, c->d_image, c->d_coeffs, height, width, width, c->d_factors, c->stream) ||
-            bdx_copy_d2h(dst, c->d_coeffs, coeff_bytes, c->stream) || bdx_stream_sync(c->stream))
+            to_host(c, dst, c->d_coeffs, coeff_bytes))
             rc = BRIDCT_DEVICE_ERROR;
     } else {
         unsigned char *img = dst;
-        if (bdx_copy_h2d(c->d_coeffs, src, coeff_bytes, c->stream) ||
+        if (to_device(c, c->d_coeffs, src, coeff_bytes) ||
             bdx_codec8(1, c->d_coeffs, c->d_image, height, width, width, c->d_factors, c->stream) ||
-            bdx_copy_d2h(packed ? packed : img, c->d_image, image_bytes, c->stream) || bdx_stream_sync(c->stream))
+            to_host(c, packed ? packed : img, c->d_image, image_bytes))
             rc = BRIDCT_DEVICE_ERROR;
         else if (packed)
             for (size_t r = 0; r < height; r++)

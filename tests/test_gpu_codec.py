@@ -125,6 +125,24 @@ def test_host_entry_points_and_strides():
         codec.close()
 
 
+def test_host_entry_points_staged():
+    """Host calls whose copies reach 1 MiB go through pinned staging in 8 MiB chunks: a
+    2048 x 3072 image (6 MiB of pixels, 12 MiB of coefficients) and a strided view of it
+    give the device path's results bitwise."""
+    big = image(2048, 3072 + 64, 11)
+    codec = capi.Codec8(QUANTS['luma'])
+    try:
+        for view in (np.ascontiguousarray(big[:, :3072]), big[:, 64:]):
+            print('staged codec host call, contiguous %s' % view.flags.c_contiguous, flush=True)
+            h, w = view.shape
+            host = codec.encode(view)
+            assert np.array_equal(host, device_encode(codec, np.ascontiguousarray(view)))
+            pix = codec.decode(host, h, w)
+            assert np.array_equal(pix, device_decode(codec, host, h, w))
+    finally:
+        codec.close()
+
+
 def test_saturation():
     codec = capi.Codec8(np.full(64, 1e-3, np.float32))
     try:
