@@ -3,9 +3,9 @@
  * A round: the caller publishes the jobs, bumps `round` and wakes the helpers; every
  * thread copies its piece of each job; the caller waits until `pending` (helpers not yet
  * done with the round) is zero.  The caller never starts a round before the previous one
- * has finished, so a helper cannot miss a round.  A helper counts rounds from 0, the
- * round current when it is created: a helper that first takes the lock after the caller
- * has already started round 1 must still take part in it. */
+ * has finished, so a helper cannot miss a round.  A helper counts rounds from zero, the
+ * count when it is created: a helper that first takes the lock after the caller has
+ * already started the first round must still take part in it. */
 #include "copy_pool.h"
 
 #include <pthread.h>
