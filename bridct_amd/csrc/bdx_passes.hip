@@ -4,6 +4,11 @@
 //   k_rows<W, OP>:  row_unit bands; rows of all arrays of a batch are one sequence.
 //   k_cols<H, MODE>: col_unit strips of one array, in place on the output; one wave per
 //                   team member (column teams), one lane per column below 128.
+//   k_cols_narrow:  the same column teams on strips of 32 or 16 columns, for grids that
+//                   would leave CUs idle.
+// Streaming calls (input plus output beyond the 256 MB Infinity Cache) use non-temporal
+// loads in the row pass, whose output stays cacheable for the column pass, and
+// non-temporal loads and stores in the column pass.
 // The gather forms (G) take array i of the batch at element offs[i] of in and out instead
 // of at i * h * w (the arrays of one shape inside a ragged batch).  A row band never
 // crosses arrays: every two-pass height is a multiple of the band.
