@@ -28,7 +28,10 @@
  * GROUPED_MIN_PIXELS pixels.  Batches of tiny arrays always use the tiny kernel.  The
  * whole-array shapes, whose launches are short and leave most of the GPU idle, go on a
  * second stream (forked from and joined back to the batch stream with events, so a
- * graph capture records both) and run beside the two-pass shapes.
+ * graph capture records both) and run beside the two-pass shapes.  GROUPED also follows
+ * the schedule table's route per shape: a shape the table sends to a split-MFMA route
+ * (both axes >= 64) runs the dense GEMMs in gather form, so the automatic choice takes
+ * GROUPED whenever such a shape is present (the persistent kernel runs butterflies only).
  *
  * The object owns its stream, device tables, item lists and counters, the grouped offset
  * table, and one captured graph (hipGraph) for replays of the same (mode, input, output);
@@ -58,11 +61,22 @@
  * small against it). */
 #define GROUPED_MIN_PIXELS ((size_t)1 << 21)
 
-/* One distinct shape of a batch: its arrays' offsets are d_goffs[first, first + count). */
+/* One distinct shape of a batch: its arrays' offsets are d_goffs[first, first + count).
+ * Dense shapes: Q_h', Q_h'^T, Q_w' at d_dense + qoff, and the exact scale 1 / (s_h s_w). */
 typedef struct {
     int h, w, pack;
+    int route;               /* BRIDCT_ROUTE_* from the schedule table */
     size_t count, first;
+    size_t qoff;
+    float dscale;
 } group;
+
+/* Arrays per launch of the dense GEMMs of an h-row shape (grid limits). */
+static size_t dense_slice(int h)
+{
+    const size_t by_rows = (size_t)65535 * 64 / (size_t)h;
+    return by_rows < 65535 ? by_rows : 65535;
+}
 
 typedef struct {
     bdx_item *items;
@@ -81,6 +95,9 @@ struct bridct_batch {
     group *groups;           /* distinct shapes, largest total first */
     unsigned ngroups;
     size_t *d_goffs;         /* element offsets of the arrays, grouped by shape */
+    float *d_dense;          /* matrices of the dense shapes (GROUPED) */
+    float *d_dwork;          /* their intermediate, one slice of the largest dense shape */
+    int ndense;
     void *aux_stream;        /* whole-array shapes of the grouped strategy */
     void *ev_fork, *ev_join;
     host_stage *stage;       /* pinned staging of large host-pointer calls, on first use */
@@ -214,7 +231,8 @@ void bridct_batch_destroy(bridct_batch *b)
     if (b->graph)
         bdx_graph_destroy(b->graph);
     stage_destroy(b->stage);
-    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr, b->d_in, b->d_out, b->d_goffs};
+    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr,
+                    b->d_in,     b->d_out,  b->d_goffs,    b->d_dense,    b->d_dwork};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             bdx_free(bufs[i]);
@@ -277,8 +295,13 @@ static int build_groups(bridct_batch *b, const int *heights, const int *widths, 
             g->h = heights[a];
             g->w = widths[a];
             g->first = i;
-            int route = 0, pack = 0;
-            g->pack = bridct_schedule_lookup(g->h, g->w, &route, &pack) == 0 ? pack : 0;
+            int route = BRIDCT_ROUTE_BUTTERFLY, pack = 0;
+            if (bridct_schedule_lookup(g->h, g->w, &route, &pack) != 0)
+                route = BRIDCT_ROUTE_BUTTERFLY, pack = 0;
+            if (route != BRIDCT_ROUTE_BUTTERFLY && (g->h < 64 || g->w < 64))
+                route = BRIDCT_ROUTE_BUTTERFLY;   /* the dense route needs both axes >= 64 */
+            g->route = route;
+            g->pack = pack;
         }
         b->groups[b->ngroups - 1].count++;
     }
@@ -292,6 +315,38 @@ static int upload(void **dst, const void *src, size_t bytes, void *stream)
     if (bdx_malloc(dst, bytes ? bytes : 16))
         return -1;
     return bytes ? bdx_copy_h2d(*dst, src, bytes, stream) : 0;
+}
+
+/* Matrices and workspace of the dense shapes (route from the schedule table). */
+static int dense_setup(bridct_batch *b)
+{
+    size_t floats = 0, work = 0;
+    for (unsigned i = 0; i < b->ngroups; i++) {
+        group *g = &b->groups[i];
+        if (g->route == BRIDCT_ROUTE_BUTTERFLY)
+            continue;
+        g->qoff = floats;
+        floats += 2 * (size_t)g->h * g->h + (size_t)g->w * g->w;
+        g->dscale = 1.0f / (float)(bridct_dense_scale(g->h) * bridct_dense_scale(g->w));
+        const size_t n = g->count < dense_slice(g->h) ? g->count : dense_slice(g->h);
+        if (n * g->h * g->w > work)
+            work = n * g->h * g->w;
+    }
+    float *host = malloc(floats * sizeof(float));
+    if (!host)
+        return -1;
+    for (unsigned i = 0; i < b->ngroups; i++) {
+        const group *g = &b->groups[i];
+        if (g->route == BRIDCT_ROUTE_BUTTERFLY)
+            continue;
+        const size_t hh = (size_t)g->h * g->h;
+        bridct_dense_matrix(g->h, host + g->qoff, host + g->qoff + hh);
+        bridct_dense_matrix(g->w, host + g->qoff + 2 * hh, NULL);
+    }
+    const int rc = upload((void **)&b->d_dense, host, floats * sizeof(float), b->stream) ||
+                   bdx_malloc((void **)&b->d_dwork, work * sizeof(float));
+    free(host);
+    return rc ? -1 : 0;
 }
 
 void bdx_schedule_free(bdx_schedule *s)
@@ -407,8 +462,11 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     size_t *goffs = malloc(b->count * sizeof(size_t));
     if (!host_tables || !zeros || !goffs || build_groups(b, heights, widths, goffs))
         goto fail;
-    b->auto_strategy = !b->tiny && b->elements / b->ngroups >= GROUPED_MIN_PIXELS ? BRIDCT_BATCH_GROUPED
-                                                                                   : BRIDCT_BATCH_PERSISTENT;
+    for (unsigned i = 0; i < b->ngroups; i++)
+        b->ndense += b->groups[i].route != BRIDCT_ROUTE_BUTTERFLY;
+    b->auto_strategy = !b->tiny && (b->ndense || b->elements / b->ngroups >= GROUPED_MIN_PIXELS)
+                           ? BRIDCT_BATCH_GROUPED
+                           : BRIDCT_BATCH_PERSISTENT;
     b->strategy = b->auto_strategy;
     bridct_build_tables(host_tables);
     if (bdx_stream_create(&b->stream) ||
@@ -419,7 +477,7 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) ||
         upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) ||
         bdx_stream_create(&b->aux_stream) || bdx_event_create(&b->ev_fork) || bdx_event_create(&b->ev_join) ||
-        bdx_stream_sync(b->stream))
+        (b->ndense && dense_setup(b)) || bdx_stream_sync(b->stream))
         goto fail;
     free(host_tables);
     free(zeros);
@@ -463,6 +521,30 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
                             mode, b->d_tables, b->blocks, b->spin_limit, flags, b->stream);
 }
 
+/* A dense shape on the batch stream, in slices of the GEMMs' grid limits; the round trip
+ * is the forward transform then the inverse in place, as on the plan path. */
+static int launch_dense(bridct_batch *b, const group *g, int mode, const float *d_in, float *d_out)
+{
+    const int prec = g->route == BRIDCT_ROUTE_MFMA_F16 ? BDX_DENSE_F16 : BDX_DENSE_BF16;
+    const size_t hh = (size_t)g->h * g->h, slice = dense_slice(g->h);
+    const float *qh = b->d_dense + g->qoff, *qht = qh + hh, *qw = qht + hh;
+    for (size_t done = 0; done < g->count; done += slice) {
+        const size_t n = g->count - done < slice ? g->count - done : slice;
+        const size_t *offs = b->d_goffs + g->first + done;
+        if (mode == BDX_ROUNDTRIP) {
+            if (bdx_dense_transform_gather(d_in, d_out, b->d_dwork, g->h, g->w, n, offs, BDX_FORWARD, prec, qh, qht,
+                                           qw, g->dscale, b->stream) ||
+                bdx_dense_transform_gather(d_out, d_out, b->d_dwork, g->h, g->w, n, offs, BDX_INVERSE, prec, qh,
+                                           qht, qw, g->dscale, b->stream))
+                return -1;
+        } else if (bdx_dense_transform_gather(d_in, d_out, b->d_dwork, g->h, g->w, n, offs, mode, prec, qh, qht, qw,
+                                              g->dscale, b->stream)) {
+            return -1;
+        }
+    }
+    return 0;
+}
+
 /* GROUPED: the plan path's kernels, per distinct shape, in gather form; two-pass shapes
  * on the batch stream, whole-array shapes on the auxiliary stream beside them. */
 static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d_out)
@@ -476,6 +558,10 @@ static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d
     int rc = 0;
     for (unsigned i = 0; i < b->ngroups && !rc; i++) {
         const group *g = &b->groups[i];
+        if (g->route != BRIDCT_ROUTE_BUTTERFLY) {
+            rc = launch_dense(b, g, mode, d_in, d_out);
+            continue;
+        }
         void *s = fork && BDX_SMALL_ROUTE(g->h, g->w) ? b->aux_stream : b->stream;
         rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
                                   b->d_tables, s);

@@ -83,10 +83,14 @@ template <typename V> __device__ __forceinline__ void stage8(const float (&x)[8]
 // Lane l stages k values 8 (l % 4) .. +7 of tile rows l / 4 + 64 j (A, B^T: two float4
 // loads each) or of tile columns l % 64 + 64 j (B: eight loads, each wave reading 64
 // consecutive floats of a row).
-template <int PREC, bool BT, int TILE>
+// GM: 0 contiguous operands; 1 (single GEMM) row m of A at A + offs[m / hrows] +
+// (m % hrows) K, i.e. the rows of the arrays of one shape inside a ragged batch; 2
+// (batched) C of batch entry z at C + offs[z].
+template <int PREC, bool BT, int TILE, int GM>
 __global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *__restrict__ A, const float *__restrict__ B,
                                               float *__restrict__ C, int N, int K, long long sa, long long sb,
-                                              long long sc, float scale)
+                                              long long sc, float scale, const size_t *__restrict__ offs,
+                                              int hrows)
 {
     using H = Half<PREC>;
     using T = typename H::T;
@@ -97,9 +101,14 @@ __global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const long long m0 = (long long)blockIdx.y * TILE, n0 = (long long)blockIdx.x * TILE;
-    A += blockIdx.z * sa + m0 * K;
+    const float *arow[S];   // A rows of this lane's stage slots
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const long long m = m0 + (tid >> 2) + 64 * j;
+        arow[j] = GM == 1 ? A + offs[m / hrows] + (m % hrows) * K : A + blockIdx.z * sa + m * K;
+    }
     B += blockIdx.z * sb;
-    C += blockIdx.z * sc;
+    C += GM == 2 ? (long long)offs[blockIdx.z] : blockIdx.z * sc;
     f16v acc[S][S] = {}, acc_lo[S][S] = {};   // hi x hi products, and the two small cross products
     const int ar = tid >> 2, ak = (tid & 3) * 8;            // A / B^T: row, first k
     const int bn = tid & 63, bk = (tid >> 6) * 8;           // B: column, first k
@@ -109,7 +118,7 @@ __global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *
     const auto load = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < S; j++) {
-            const float *src = A + (long long)(ar + 64 * j) * K + k0 + ak;
+            const float *src = arow[j] + k0 + ak;
             const float4 u = *reinterpret_cast<const float4 *>(src);
             const float4 v = *reinterpret_cast<const float4 *>(src + 4);
             xa[j][0] = u.x, xa[j][1] = u.y, xa[j][2] = u.z, xa[j][3] = u.w;
@@ -182,39 +191,56 @@ __global__ __launch_bounds__(256, TILE == 128 ? 2 : 1) void k_gemm(const float *
         }
 }
 
-template <int PREC, int TILE>
+template <int PREC, int TILE, int GM>
 void launch(bool bt, dim3 grid, hipStream_t s, const float *A, const float *B, float *C, int N, int K, long long sa,
-            long long sb, long long sc, float scale)
+            long long sb, long long sc, float scale, const size_t *offs, int hrows)
 {
     if (bt)
-        hipLaunchKernelGGL((k_gemm<PREC, true, TILE>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale);
+        hipLaunchKernelGGL((k_gemm<PREC, true, TILE, GM>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale,
+                           offs, hrows);
     else
-        hipLaunchKernelGGL((k_gemm<PREC, false, TILE>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale);
+        hipLaunchKernelGGL((k_gemm<PREC, false, TILE, GM>), grid, dim3(256), 0, s, A, B, C, N, K, sa, sb, sc, scale,
+                           offs, hrows);
+}
+
+template <int PREC, int TILE>
+void launch_gm(int gm, bool bt, dim3 grid, hipStream_t s, const float *A, const float *B, float *C, int N, int K,
+               long long sa, long long sb, long long sc, float scale, const size_t *offs, int hrows)
+{
+    if (gm == 1)
+        launch<PREC, TILE, 1>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale, offs, hrows);
+    else if (gm == 2)
+        launch<PREC, TILE, 2>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale, offs, hrows);
+    else
+        launch<PREC, TILE, 0>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale, offs, hrows);
 }
 
 template <int PREC>
 int gemm(bool bt, const float *A, const float *B, float *C, int M, int N, int K, int batch, long long sa,
-         long long sb, long long sc, float scale, hipStream_t s)
+         long long sb, long long sc, float scale, hipStream_t s, int gm, const size_t *offs, int hrows)
 {
     // 128 x 128 tiles where both dimensions allow, else 64 x 64
     const int tile = M % 128 == 0 && N % 128 == 0 ? 128 : 64;
     if (M % 64 || N % 64 || K % BK || batch < 1 || batch > 65535 || M / tile > 65535 || N / tile > 65535)
         return -1;
     const dim3 grid((unsigned)(N / tile), (unsigned)(M / tile), (unsigned)batch);
+    if (gm == 1 && (hrows < 1 || M % hrows))
+        return -1;
     if (tile == 128)
-        launch<PREC, 128>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale);
+        launch_gm<PREC, 128>(gm, bt, grid, s, A, B, C, N, K, sa, sb, sc, scale, offs, hrows);
     else
-        launch<PREC, 64>(bt, grid, s, A, B, C, N, K, sa, sb, sc, scale);
+        launch_gm<PREC, 64>(gm, bt, grid, s, A, B, C, N, K, sa, sb, sc, scale, offs, hrows);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int gemm_any(int prec, bool bt, const float *A, const float *B, float *C, int M, int N, int K, int batch,
-             long long sa, long long sb, long long sc, float scale, hipStream_t s)
+             long long sa, long long sb, long long sc, float scale, hipStream_t s, int gm = 0,
+             const size_t *offs = nullptr, int hrows = 0)
 {
     if (prec == BDX_DENSE_F16)
-        return gemm<BDX_DENSE_F16>(bt, A, B, C, M, N, K, batch, sa, sb, sc, scale, s);
+        return gemm<BDX_DENSE_F16>(bt, A, B, C, M, N, K, batch, sa, sb, sc, scale, s, gm, offs, hrows);
     if (prec == BDX_DENSE_BF16)
-        return gemm<BDX_DENSE_BF16>(bt, A, B, C, M, N, K, batch, sa, sb, sc, scale, s);
+        return gemm<BDX_DENSE_BF16>(bt, A, B, C, M, N, K, batch, sa, sb, sc, scale, s, gm, offs, hrows);
     return -1;
 }
 
@@ -238,4 +264,24 @@ extern "C" int bdx_dense_transform(const float *d_in, float *d_out, float *d_wor
     // columns: Y = Q_h' T (forward) or X = Q_h'^T T (inverse), per array
     return gemm_any(prec, false, mode == BDX_FORWARD ? d_qh : d_qht, d_work, d_out, h, w, h, (int)batch, 0, hw,
                     hw, scale, s);
+}
+
+extern "C" int bdx_dense_transform_gather(const float *d_in, float *d_out, float *d_work, int h, int w, size_t count,
+                                          const size_t *d_offsets, int mode, int prec, const float *d_qh,
+                                          const float *d_qht, const float *d_qw, float scale, void *stream)
+{
+    using namespace bdx;
+    if (count == 0)
+        return 0;
+    if (h < 64 || w < 64 || (mode != BDX_FORWARD && mode != BDX_INVERSE) || count > 65535 || !d_offsets)
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const long long hw = (long long)h * w;
+    // rows, gathered through the offsets: T = X Q_w'^T (forward) or T = Y Q_w' (inverse)
+    if (gemm_any(prec, mode == BDX_FORWARD, d_in, d_qw, d_work, (int)(count * h), w, w, 1, 0, 0, 0, 1.0f, s, 1,
+                 d_offsets, h))
+        return -1;
+    // columns, per array, scattered through the offsets
+    return gemm_any(prec, false, mode == BDX_FORWARD ? d_qh : d_qht, d_work, d_out, h, w, h, (int)count, 0, hw, 0,
+                    scale, s, 2, d_offsets, h);
 }

@@ -167,6 +167,13 @@ int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, i
                         int prec, const float *d_qh, const float *d_qht, const float *d_qw, float scale,
                         void *stream);
 
+/* The same for `count` h x w arrays at the element offsets d_offsets[0, count) of d_in and
+ * d_out (the arrays of one shape inside a ragged batch); count <= 65535 and
+ * count h / 64 <= 65535; d_work: count h w floats, contiguous. */
+int bdx_dense_transform_gather(const float *d_in, float *d_out, float *d_work, int h, int w, size_t count,
+                               const size_t *d_offsets, int mode, int prec, const float *d_qh, const float *d_qht,
+                               const float *d_qw, float scale, void *stream);
+
 /* 8x8 block codec step over a height x width u8 raster (multiples of 8; row stride
  * `stride` bytes, a multiple of 8; 8-byte aligned) and its block-contiguous int16
  * coefficients (64 per block, 16-byte aligned).  decode = 0: d_src is the raster, d_dst the

@@ -11,6 +11,11 @@ __attribute__((visibility("hidden"))) void bridct_build_tables(float *t);
  * device tables hold rounded to FP32). */
 __attribute__((visibility("hidden"))) double bridct_axis_alpha(int n, int k);
 
+/* Dense route: s_n = 2^floor(log2 sqrt n), and Q_n' = s_n Q_n (n x n, row-major, FP64
+ * rounded once) with, when qt is not NULL, its transpose. */
+__attribute__((visibility("hidden"))) int bridct_dense_scale(int n);
+__attribute__((visibility("hidden"))) void bridct_dense_matrix(int n, float *q, float *qt);
+
 /* Route and whole-array packing of shape (h, w) from the schedule table; 0 if found. */
 __attribute__((visibility("hidden"))) int bridct_schedule_lookup(int h, int w, int *route, int *pack);
 

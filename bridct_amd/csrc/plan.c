@@ -231,7 +231,7 @@ static int distance_below(const void *a, const void *b, size_t bytes)
 
 /* Power of two near sqrt(n) (2^floor(log2 sqrt n)): the dense route scales Q_n by it so
  * that the lo parts of the split matrix entries stay normal FP16 numbers. */
-static int dense_matrix_scale(int n)
+int bridct_dense_scale(int n)
 {
     int s = 1;
     while ((long)(2 * s) * (2 * s) <= n)
@@ -242,9 +242,9 @@ static int dense_matrix_scale(int n)
 /* Q_n'[k][j] = s_n sqrt((k ? 2 : 1) / n) cos(pi k (2j + 1) / (2n)) in FP64, rounded once;
  * the cosine argument k (2j + 1) is reduced modulo 4n first.  Writes Q' and, when qt is
  * not NULL, its transpose. */
-static void dense_matrix(int n, float *q, float *qt)
+void bridct_dense_matrix(int n, float *q, float *qt)
 {
-    const double s = dense_matrix_scale(n);
+    const double s = bridct_dense_scale(n);
     for (int k = 0; k < n; k++) {
         const double f = s * sqrt((k ? 2.0 : 1.0) / n);
         for (int j = 0; j < n; j++) {
@@ -268,8 +268,8 @@ int bridct_plan_set_route(bridct_plan *plan, int route)
         float *host = malloc((2 * hh + ww) * sizeof(float));
         if (!host)
             return BRIDCT_DEVICE_ERROR;
-        dense_matrix(plan->height, host, host + hh);
-        dense_matrix(plan->width, host + 2 * hh, NULL);
+        bridct_dense_matrix(plan->height, host, host + hh);
+        bridct_dense_matrix(plan->width, host + 2 * hh, NULL);
         if (bdx_malloc((void **)&plan->d_dense, (2 * hh + ww) * sizeof(float)) ||
             bdx_copy_h2d(plan->d_dense, host, (2 * hh + ww) * sizeof(float), plan->stream) ||
             bdx_stream_sync(plan->stream)) {
@@ -280,7 +280,7 @@ int bridct_plan_set_route(bridct_plan *plan, int route)
             return BRIDCT_DEVICE_ERROR;
         }
         free(host);
-        plan->dense_scale = 1.0f / (float)(dense_matrix_scale(plan->height) * dense_matrix_scale(plan->width));
+        plan->dense_scale = 1.0f / (float)(bridct_dense_scale(plan->height) * bridct_dense_scale(plan->width));
     }
     plan->route = route;
     return BRIDCT_OK;

@@ -150,12 +150,14 @@ BRIDCT_API int bridct_codec8_synchronize(bridct_codec8 *codec);
 BRIDCT_API void *bridct_codec8_stream(const bridct_codec8 *codec);
 
 /* Execution strategy of a batch.  PERSISTENT: one persistent kernel launch per direction
- * (the tiny kernel when every array is at most 32 x 32).  GROUPED: the plan path's kernels
- * per distinct shape, reading the arrays of that shape through an offset table, one or two
- * launches per shape on the batch stream (one graph with bridct_batch_graph_apply); its
- * results are bitwise those of per-shape plans.  AUTO (the default at creation): GROUPED
- * when the batch averages at least 2^21 pixels per distinct shape and is not tiny, else
- * PERSISTENT.  get returns the strategy in effect (never AUTO). */
+ * (the tiny kernel when every array is at most 32 x 32), butterfly route only.  GROUPED:
+ * the plan path's kernels per distinct shape, reading the arrays of that shape through an
+ * offset table, one or two launches per shape on the batch stream (one graph with
+ * bridct_batch_graph_apply), with the route the schedule table gives each shape (the
+ * split-MFMA GEMMs for a dense route, both axes >= 64); its results are bitwise those of
+ * per-shape plans.  AUTO (the default at creation): GROUPED when the batch is not tiny and
+ * either averages at least 2^21 pixels per distinct shape or has a shape the table sends
+ * to a dense route, else PERSISTENT.  get returns the strategy in effect (never AUTO). */
 #define BRIDCT_BATCH_AUTO 0
 #define BRIDCT_BATCH_PERSISTENT 1
 #define BRIDCT_BATCH_GROUPED 2

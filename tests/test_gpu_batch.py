@@ -319,3 +319,44 @@ def test_wait_bound_reports_error_and_recovers():
         assert np.abs(y - r).max() <= 1e-6 * np.abs(r).max()
     finally:
         b.close()
+
+
+def test_grouped_follows_table_routes(tmp_path, monkeypatch):
+    """A schedule table that sends shapes to the split-MFMA routes: plans take those routes,
+    a grouped batch runs the same dense GEMMs in gather form and equals the per-shape plans
+    bitwise (eager and graph replay, every mode), the automatic strategy is GROUPED, and
+    the persistent kernel, which runs butterflies only, agrees with butterfly plans to FP32
+    rounding.  A dense entry for a shape with an axis below 64 is ignored by both.  (The
+    split routes' own error grows for arrays far below unit scale, where the FP16 lo halves
+    become subnormal; the dense tests cover it.)"""
+    table = tmp_path / 'schedule.txt'
+    table.write_text('64 64 mfma_f16 0\n128 256 mfma_f16 0\n64 1024 mfma_bf16 0\n512 64 mfma_f16 0\n'
+                     '32 64 mfma_f16 0\n')
+    monkeypatch.setenv('BRIDCT_SCHEDULE', str(table))
+    shapes = ([(64, 64)] * 5 + [(128, 256)] * 3 + [(16, 16)] * 4 + [(64, 1024)] * 2 + [(512, 64)] * 2 +
+              [(32, 64)] * 3 + [(256, 256)] * 2)
+    arrays = ragged(shapes, 11)
+    x = flat(arrays)
+    for (h, w), route in (((64, 64), capi.ROUTE_MFMA_F16), ((64, 1024), capi.ROUTE_MFMA_BF16),
+                          ((32, 64), capi.ROUTE_BUTTERFLY), ((256, 256), capi.ROUTE_BUTTERFLY)):
+        p = capi.Plan(h, w)
+        assert p.route == route, (h, w)
+        p.close()
+    b = capi.Batch(shapes)
+    try:
+        assert b.strategy == capi.BATCH_GROUPED
+        for mode in (0, 1, 2, FTI):
+            print('dense routes in a grouped batch, mode %d' % mode, flush=True)
+            if mode == FTI:
+                ref = flat(plan_results(shapes, plan_results(shapes, arrays, 0), 1))
+            else:
+                ref = flat(plan_results(shapes, arrays, mode))
+            assert np.array_equal(run_device(b, mode, x), ref)
+            assert np.array_equal(run_device(b, mode, x, graph=True), ref)
+        assert b.set_strategy(capi.BATCH_PERSISTENT) == 0
+        monkeypatch.setenv('BRIDCT_SCHEDULE', 'none')   # butterfly plans for the comparison
+        for mode in (0, 1):
+            y = run_device(b, mode, x)
+            assert agree(y, flat(plan_results(shapes, arrays, mode)), capi.BATCH_PERSISTENT)
+    finally:
+        b.close()
