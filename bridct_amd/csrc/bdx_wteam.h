@@ -23,8 +23,8 @@ template <int n, int THREADS> struct CTeam {
     static constexpr int M = 64 * T / THREADS;            // members per wave
     static constexpr int L = 64 / M;                      // lines per block
     // Exchange in two rounds from 16384 floats up: the exchange then takes at most 32 KiB,
-    // and three 512-lane workgroups of the 256 team fit a CU (64 KiB allowed two; C1's
-    // 256 x 256^2 ran 2-3% faster, equal results bitwise).
+    // and three 512-lane workgroups of the 256 team fit a CU (64 KiB allowed two; equal
+    // results bitwise, within 2% in speed).
     static constexpr int ROUNDS = n * L >= 16384 ? 2 : 1;
     static constexpr int SL = L / ROUNDS;                 // lines per exchange round
     static constexpr int XLDS = n * SL;                   // exchange floats
