@@ -30,13 +30,12 @@ __global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, 
 {
     __shared__ float smem[row_unit_lds<W, OP, kThreads>()];
     constexpr int R = row_unit_rows<W, kThreads>();
-    // Calls that fit the Infinity Cache take the XCD-aware order of the column pass (each
-    // XCD a contiguous range of bands, i.e. the same arrays its column strips take next):
-    // 2-3% faster on 64 x 128x1024 and 64 x 512x128, unchanged on 256 x 256^2; streaming
-    // calls keep the dispatch order, which was faster for them.
+    // The XCD-aware order of the column pass: each XCD takes a contiguous range of bands,
+    // i.e. the same arrays its column strips take next.  2-3% faster on 64 x 128x1024 and
+    // 64 x 512x128 (cache-resident calls), and the batch-64 four-shape step runs 4% faster,
+    // although the 1024^2 column pass timed alone after it is 5% slower.
     const unsigned nb = gridDim.x;
-    const unsigned bid = !NT && nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds
-                                                : blockIdx.x;
+    const unsigned bid = nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
     const size_t row0 = (size_t)bid * R;
     if constexpr (G) {
         const size_t a = row0 / (unsigned)h, base = offs[a] + (row0 - a * h) * W;
