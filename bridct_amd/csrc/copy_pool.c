@@ -117,8 +117,11 @@ void pool_copy(copy_pool *p, const copy_job *jobs, int njobs)
 {
     if (njobs <= 0)
         return;
-    if (njobs > 2)
-        njobs = 2;
+    if (njobs > 2) {   /* one round per pair of jobs */
+        for (int j = 0; j < njobs; j += 2)
+            pool_copy(p, jobs + j, njobs - j < 2 ? njobs - j : 2);
+        return;
+    }
     pthread_mutex_lock(&p->m);
     for (int j = 0; j < njobs; j++)
         p->jobs[j] = jobs[j];

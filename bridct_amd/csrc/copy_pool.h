@@ -23,8 +23,8 @@ __attribute__((visibility("hidden"))) copy_pool *pool_create(void);
 /* Stops and joins the helpers; NULL is a no-op. */
 __attribute__((visibility("hidden"))) void pool_destroy(copy_pool *p);
 
-/* Copies the jobs (at most 2), each split into one contiguous piece per thread, and
- * returns when every piece is done. */
+/* Copies the jobs, each split into one contiguous piece per thread (two jobs per round),
+ * and returns when every piece is done. */
 __attribute__((visibility("hidden"))) void pool_copy(copy_pool *p, const copy_job *jobs, int njobs);
 
 /* Helpers running (for tests). */

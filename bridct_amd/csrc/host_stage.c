@@ -16,6 +16,8 @@ struct host_stage {
 
 host_stage *stage_create(size_t chunk_bytes)
 {
+    if (!chunk_bytes)
+        return NULL;
     host_stage *s = calloc(1, sizeof(*s));
     if (!s)
         return NULL;

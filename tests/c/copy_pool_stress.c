@@ -1,6 +1,6 @@
 /* Stress driver of the host copy pool (bridct_amd/csrc/copy_pool.c), CPU only.
  * usage: copy_pool_stress POOLS ROUNDS MAXBYTES
- * Creates POOLS pools one after another; each runs ROUNDS rounds of one or two jobs of
+ * Creates POOLS pools one after another; each runs ROUNDS calls of one to four jobs of
  * random sizes below MAXBYTES right after creation (the helpers' start-up race), checks
  * every copied byte, and is destroyed.  Prints "ok <helpers>" on success. */
 #include "copy_pool.h"
@@ -21,8 +21,8 @@ int main(int argc, char **argv)
         return 2;
     const int pools = atoi(argv[1]), rounds = atoi(argv[2]);
     const size_t cap = (size_t)atol(argv[3]);
-    unsigned char *src[2], *dst[2];
-    for (int j = 0; j < 2; j++) {
+    unsigned char *src[4], *dst[4];
+    for (int j = 0; j < 4; j++) {
         src[j] = malloc(cap ? cap : 1);
         dst[j] = malloc(cap ? cap : 1);
         if (!src[j] || !dst[j])
@@ -36,8 +36,8 @@ int main(int argc, char **argv)
             return 3;
         helpers = pool_helpers(p);
         for (int r = 0; r < rounds; r++) {
-            const int nj = 1 + (int)(next(&seed) & 1);
-            copy_job jobs[2];
+            const int nj = 1 + (int)(next(&seed) & 3);
+            copy_job jobs[4];
             for (int j = 0; j < nj; j++) {
                 const size_t n = cap ? next(&seed) % (cap + 1) : 0;
                 const unsigned char tag = (unsigned char)next(&seed);
