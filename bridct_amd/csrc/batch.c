@@ -343,8 +343,9 @@ static int dense_setup(bridct_batch *b)
         bridct_dense_matrix(g->h, host + g->qoff, host + g->qoff + hh);
         bridct_dense_matrix(g->w, host + g->qoff + 2 * hh, NULL);
     }
+    /* the copy from pageable memory must be done before `host` is freed */
     const int rc = upload((void **)&b->d_dense, host, floats * sizeof(float), b->stream) ||
-                   bdx_malloc((void **)&b->d_dwork, work * sizeof(float));
+                   bdx_stream_sync(b->stream) || bdx_malloc((void **)&b->d_dwork, work * sizeof(float));
     free(host);
     return rc ? -1 : 0;
 }
