@@ -541,7 +541,7 @@ def main():
     h, w, plan, (a, b, c) = runs[0]
     px = 1024 * 1024 * BATCH
     kernels = {}
-    reps = 6
+    reps = 21   # 20 timed dispatches of each kernel
     for mode in (capi.FORWARD, capi.INVERSE):
         ev = [hip.event() for _ in range(2 * reps + 1)]
         hip.record(ev[0], plan.stream)
