@@ -153,9 +153,25 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         const float *src = in + base + c0;
         float *dst = out + base + c0;
         const bool live = line < ncols;
-        const auto ld = [&](int i) { return live ? ld1<NT>(src + (unsigned)(i * w) + line) : 0.0f; };
+        // Byte offsets from the strip base in 32 bits (an array is at most 4 MiB): the loads
+        // and stores then take the uniform base plus a 32-bit lane offset, with no 64-bit
+        // address add per access (length 1024: 23% fewer non-arithmetic vector instructions,
+        // 140 / 149 VGPRs instead of 142 / 154, and the round trip no longer spills).  The
+        // length-512 round trip keeps element offsets: in this form its registers rose from
+        // 137 to 252 and it ran 4% slower.  Equal addresses either way.
+        constexpr bool kByteOff = !(H == 512 && MODE == 2);
+        const auto ld = [&](int i) {
+            if constexpr (kByteOff)
+                return live ? ld1<NT>((const float *)((const char *)src + (unsigned)(i * w + line) * 4u)) : 0.0f;
+            else
+                return live ? ld1<NT>(src + (unsigned)(i * w) + line) : 0.0f;
+        };
         const auto st = [&](int i, float v) {
-            if (live)
+            if (!live)
+                return;
+            if constexpr (kByteOff)
+                st1<NT>((float *)((char *)dst + (unsigned)(i * w + line) * 4u), v);
+            else
                 st1<NT>(dst + (unsigned)(i * w) + line, v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
