@@ -19,7 +19,15 @@ library compiled from its own sources by oracle/Makefile (oracle/_ref/libbridct.
   (dataset/evaluate.py:17-21), and the reference's published maximum over core cases
   (dataset/DATASET_CARD.md:34) for comparison.
 * corpus_manifest.json: id, size, domain and sha256 of every corpus array
-  (dataset/manifest.json), which pins the corpus generator of tests/corpus_gen.py.
+  (dataset/manifest.json).
+* corpus.npz: the 474 corpus arrays themselves, copied from dataset/inputs.npz after the
+  archive's and every array's sha256 were checked against dataset/manifest.json (the rule
+  of dataset/verify.py:8-28).  Tests check each array's sha256 again on load.
+* norm_tables.npz: the reference's baked normalization and correction tables, read as text
+  from src/generated/tables.inc:3-4,7-8 (norm_sj8/16, corr_sj8/16) and
+  src/generated/large.c:13574-13581 (norm_sj32..256, corr_sj32..256), each literal rounded
+  to float32 as the C compiler rounds it.  Complete for n <= 64; rows 0, 1, 2, 3, n/2 and
+  n-1 of the larger tables.  They pin oracle_norm_tables.
 """
 import argparse
 import ctypes
@@ -48,6 +56,27 @@ def parse_scales(ref):
     return out
 
 
+NORM_ROWS = lambda n: sorted({0, 1, 2, 3, n // 2, n - 1})
+
+
+def parse_norm_tables(ref):
+    """{'norm8': float32 (n, n) or the sampled rows, ...} from the reference literals."""
+    gen = os.path.join(ref, 'src/generated')
+    texts = [open(os.path.join(gen, 'tables.inc')).read(), open(os.path.join(gen, 'large.c')).read()]
+    out = {}
+    for kind in ('norm', 'corr'):
+        for n in (8, 16, 32, 64, 128, 256):
+            found = [m for t in texts for m in re.finditer(r'static const float %s_sj%d\[\]=\{([^}]*)\}' % (kind, n), t)]
+            assert len(found) == 1, (kind, n)
+            vals = np.array([np.float32(v.strip().rstrip('f')) for v in found[0].group(1).split(',')], np.float32)
+            assert vals.size == n * n, (kind, n, vals.size)
+            table = vals.reshape(n, n)
+            if n > 64:
+                table = table[NORM_ROWS(n)]
+            out['%s%d' % (kind, n)] = table
+    return out
+
+
 def error_norm(actual, expected, input_norm):
     """dataset/evaluate.py:17-21 restated."""
     if not np.isfinite(actual).all():
@@ -67,6 +96,7 @@ def main():
     scales = parse_scales(args.reference)
     with open(os.path.join(HERE, 'axis_scale.json'), 'w') as f:
         json.dump({'source': 'src/generated/rect_generated.inc scale{n}[]', 'scale': scales}, f)
+    np.savez_compressed(os.path.join(HERE, 'norm_tables.npz'), **parse_norm_tables(args.reference))
 
     ds = os.path.join(args.reference, 'dataset')
     manifest = json.load(open(os.path.join(ds, 'manifest.json')))
@@ -74,10 +104,14 @@ def main():
     assert hashlib.sha256(open(archive, 'rb').read()).hexdigest() == manifest['archive_sha256']
     rows = []
     small = {}
+    arrays = {}
     with np.load(archive, allow_pickle=False) as data:
+        assert set(data.files) == {c['id'] for c in manifest['cases']}
         for case in manifest['cases']:
             x = data[case['id']]
+            assert x.shape == (case['N'], case['N']) and x.dtype == np.dtype('<f4') and np.isfinite(x).all()
             assert hashlib.sha256(x.tobytes(order='C')).hexdigest() == case['sha256']
+            arrays[case['id']] = x
             n = case['N']
             q = S.qmatrix(n)
             xd = x.astype(np.float64)
@@ -107,6 +141,7 @@ def main():
         json.dump({'version': manifest['version'], 'archive_sha256': manifest['archive_sha256'],
                    'cases': [dict(id=c['id'], N=c['N'], domain=c['domain'], sha256=c['sha256'])
                              for c in manifest['cases']]}, f, indent=0)
+    np.savez_compressed(os.path.join(HERE, 'corpus.npz'), **arrays)
     ids = sorted(small)
     np.savez_compressed(
         os.path.join(HERE, 'corpus_small.npz'),

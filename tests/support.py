@@ -132,6 +132,41 @@ def rel_l2(y, ref):
     return d / n if n else d
 
 
+def error_norm(actual, expected, input_norm):
+    """Input-relative Frobenius error, None when the output is not finite (the metric of
+    the reference's corpus evaluation, dataset/evaluate.py:17-21)."""
+    if not np.isfinite(actual).all():
+        return None
+    error = float(np.linalg.norm(actual.astype(np.float64) - expected))
+    return error / input_norm if input_norm else (0.0 if error == 0.0 else None)
+
+
+CORPUS_SIZES = (8, 16, 32, 64, 128, 256)
+_corpus = None
+
+
+def corpus_cases(n=None):
+    """The reference corpus (474 float32 square arrays, dataset/inputs.npz) as committed in
+    tests/golden/corpus.npz: [(id, domain, array)] in manifest order, optionally only size
+    n.  Every array's sha256 is checked against the reference manifest on first load."""
+    global _corpus
+    if _corpus is None:
+        import hashlib
+        import json
+        gold = os.path.join(ROOT, 'tests', 'golden')
+        manifest = json.load(open(os.path.join(gold, 'corpus_manifest.json')))
+        cases = []
+        with np.load(os.path.join(gold, 'corpus.npz'), allow_pickle=False) as data:
+            assert set(data.files) == {c['id'] for c in manifest['cases']}
+            for c in manifest['cases']:
+                x = data[c['id']]
+                assert x.shape == (c['N'], c['N']) and x.dtype == np.dtype('<f4')
+                assert hashlib.sha256(x.tobytes(order='C')).hexdigest() == c['sha256'], c['id']
+                cases.append((c['id'], c['domain'], np.ascontiguousarray(x)))
+        _corpus = cases
+    return [c for c in _corpus if n is None or c[2].shape[0] == n]
+
+
 def families(h, w, rng):
     """The reference's shape-sweep families (tests/verify_portability.py:64-73)."""
     noise = rng.normal(size=(h, w)).astype(np.float32)

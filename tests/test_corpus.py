@@ -1,22 +1,21 @@
-"""CPU checks of the corpus harness: the regenerated corpus is the published one (sha256 of
-every array against the reference manifest, dataset/verify.py:8-28), and the error metric
-detects the reference's 78 seeded faults (dataset/evaluate.py:79-87)."""
+"""CPU checks of the corpus harness: the committed corpus arrays are the published ones
+(sha256 of every array against the reference manifest, the rule of dataset/verify.py:8-28),
+and the error metric detects the reference's 78 seeded faults (dataset/evaluate.py:79-87)."""
 import hashlib
 import json
 import os
 
 import numpy as np
 
-import corpus_gen as C
 import support as S
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 THRESHOLD = 2e-5
 
 
-def test_generator_reproduces_published_corpus():
+def test_committed_corpus_is_published_corpus():
     manifest = json.load(open(os.path.join(GOLD, 'corpus_manifest.json')))
-    cases = C.all_cases()
+    cases = S.corpus_cases()
     assert len(cases) == len(manifest['cases']) == 474
     for (cid, domain, x), m in zip(cases, manifest['cases']):
         assert cid == m['id'] and domain == m['domain'] and x.shape == (m['N'], m['N'])
@@ -28,9 +27,9 @@ def test_seeded_faults_detected():
     """For every size, mode and fault (scale by 1.01, transpose, sign, zero; lost AC for
     the forward), at least one core case must be flagged by the metric."""
     detected = {}
-    for n in C.SIZES:
+    for n in S.CORPUS_SIZES:
         q = S.qmatrix(n)
-        for cid, domain, x in C.corpus(n):
+        for cid, domain, x in S.corpus_cases(n):
             if domain != 'core':
                 continue
             xd = x.astype(np.float64)
@@ -43,7 +42,7 @@ def test_seeded_faults_detected():
                     lost[0, 0] = expected[0, 0]
                     faults['lost_ac'] = lost
                 for name, changed in faults.items():
-                    e = C.error_norm(changed, expected, norm)
+                    e = S.error_norm(changed, expected, norm)
                     key = (n, mode, name)
                     detected[key] = detected.get(key, 0) + int(e is None or e >= THRESHOLD)
     assert len(detected) == 78

@@ -1,4 +1,5 @@
-"""The reference's adversarial corpus (474 arrays x 3 directions) through the GPU plan path
+"""The reference's adversarial corpus (474 arrays x 3 directions, committed as data in
+tests/golden/corpus.npz) through the GPU plan path
 with the reference's evaluation rules (dataset/evaluate.py:55-78): return code, input
 unchanged, guard sentinels intact, input-relative Frobenius error < 2e-5.  Gate (the
 "no CPU pass turns into a GPU failure" rule of the reference's numerical comparisons):
@@ -11,7 +12,6 @@ import os
 import numpy as np
 import pytest
 
-import corpus_gen as C
 import support as S
 from bridct_amd import capi
 
@@ -24,13 +24,13 @@ def test_corpus_gate():
     L = capi.load()
     cpu = {(r['id'], r['mode']): r['error'] for r in json.load(open(os.path.join(GOLD, 'corpus_summary.json')))['rows']}
     rows = []
-    for n in C.SIZES:
+    for n in S.CORPUS_SIZES:
         print('corpus size', n, flush=True)
         q = S.qmatrix(n)
         plan = L.bridct_plan_create(n, n)
         assert plan
         try:
-            for cid, domain, x in C.corpus(n):
+            for cid, domain, x in S.corpus_cases(n):
                 original = x.view(np.uint32).copy()
                 xd = x.astype(np.float64)
                 norm = float(np.linalg.norm(xd))
@@ -40,7 +40,7 @@ def test_corpus_gate():
                     y.fill(np.nan)
                     rc = L.bridct_plan_apply(plan, mode, capi.ptr(x), capi.ptr(y))
                     with np.errstate(all='ignore'):
-                        err = C.error_norm(y, expected, norm)
+                        err = S.error_norm(y, expected, norm)
                     ok = bool(rc == 0 and np.array_equal(x.view(np.uint32), original) and
                               np.all(guard[:4] == 531.25) and np.all(guard[-4:] == 531.25) and
                               err is not None and err < THRESHOLD)

@@ -6,7 +6,11 @@ index, a transposed operand, a swapped sine/cosine branch in s_{N,k}) fails:
   * the closed form against SciPy's independent DCT-II/III (scipy.fft.dctn/idctn);
   * the C loop against the matrix form, orthogonality, impulse -> basis outer product,
     ones -> sqrt(hw) at DC, the scaled 1D core F_n against its definition;
-  * the reference corpus summary: the oracle reproduces the published error figures.
+  * the reference corpus summary: the oracle reproduces the published error figures;
+  * oracle_supported by brute force over every (h, w) in [-8, 2048]^2 against the shape
+    rule of src/plan.c:33-37 written out as a set;
+  * oracle_norm_tables against the reference's baked norm_sj / corr_sj literals
+    (src/generated/tables.inc:3-4,7-8, src/generated/large.c:13574-13581).
 """
 import json
 import os
@@ -130,3 +134,68 @@ def test_planted_mistakes_are_caught():
     bad = qh.copy()
     bad[1:] /= np.sqrt(2)
     assert np.abs(bad @ x @ qw.T - good).max() > 1e-3         # dropped sqrt(2)
+
+
+def test_supported_brute_force():
+    """Every (h, w) in [-8, 2048]^2: supported exactly when both axes are in
+    {8, 16, ..., 1024} (src/plan.c:33-37).  A plausible slip (n < 1024, a missing
+    power-of-two test, 4 admitted) fails at one of the grid's points."""
+    import ctypes
+    lib = S.oracle()
+    fn = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_int)(('oracle_supported', lib))
+    allowed = {8, 16, 32, 64, 128, 256, 512, 1024}
+    rng = range(-8, 2049)
+    ok_h = [h in allowed for h in rng]
+    count = 0
+    for h, hok in zip(rng, ok_h):
+        if hok:
+            got = [fn(h, w) for w in rng]
+            assert got == [1 if w in allowed else 0 for w in rng], h
+            count += sum(got)
+        else:
+            assert not any(fn(h, w) for w in rng), h
+    assert count == 64
+
+
+def test_norm_tables_match_reference_literals():
+    """oracle_norm_tables(h, w) against the reference's baked float tables: the
+    normalization nu_h[r] nu_w[c] and its square, row-major, to within one float32
+    rounding (the literals are the reference's own FP64 values rounded by the C
+    compiler; the oracle rounds its FP64 product once)."""
+    z = np.load(os.path.join(GOLD, 'norm_tables.npz'))
+    lib = S.oracle()
+    for n in (8, 16, 32, 64, 128, 256):
+        norm = np.empty((n, n), np.float32)
+        corr = np.empty((n, n), np.float32)
+        assert lib.oracle_norm_tables(n, n, norm.ctypes.data, corr.ctypes.data) == 0
+        rows = list(range(n)) if n <= 64 else sorted({0, 1, 2, 3, n // 2, n - 1})
+        for name, ours in (('norm', norm), ('corr', corr)):
+            ref = z['%s%d' % (name, n)]
+            assert ref.shape == (len(rows), n)
+            got = ours[rows]
+            ulp = np.spacing(np.abs(ref))
+            assert (np.abs(got - ref) <= ulp).all(), (name, n)
+            assert np.mean(got == ref) > 0.95, (name, n)
+    # rectangular plans: the outer product of the two axes' square-table factors
+    h, w = 8, 32
+    norm = np.empty((h, w), np.float32)
+    corr = np.empty((h, w), np.float32)
+    assert lib.oracle_norm_tables(h, w, norm.ctypes.data, corr.ctypes.data) == 0
+    nu_h = np.sqrt(z['norm8'].astype(np.float64).diagonal())
+    nu_w = np.sqrt(z['norm32'].astype(np.float64).diagonal())
+    assert np.allclose(norm, np.outer(nu_h, nu_w), rtol=3e-7, atol=0)
+    assert np.allclose(corr, np.outer(nu_h, nu_w) ** 2, rtol=5e-7, atol=0)
+    # unsupported shapes are rejected and write nothing
+    sentinel = np.full(4, 7.0, np.float32)
+    assert lib.oracle_norm_tables(8, 12, sentinel.ctypes.data, sentinel.ctypes.data) == -1
+    assert (sentinel == 7.0).all()
+
+
+def test_norm_tables_planted_mistakes_fail():
+    """The literal check rejects a transposed or unsquared table (shown on the CPU copy)."""
+    z = np.load(os.path.join(GOLD, 'norm_tables.npz'))
+    ref = z['norm16'].astype(np.float64)
+    nu = np.sqrt(ref.diagonal())
+    skew = np.outer(nu, nu * np.linspace(1, 1.001, 16))          # a column-dependent slip
+    assert np.abs(skew - ref).max() > np.spacing(np.float32(ref.max()))
+    assert np.abs(z['corr16'] - ref).max() > 1e-3                 # corr is the square, not norm
