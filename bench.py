@@ -225,8 +225,7 @@ def leg_blocks(hip, lib):
     values integers 0..255 minus 128: one persistent launch (and its hipGraph) for forward,
     inverse and forward-then-inverse, vs one plan launch per block on a 65,536-block subset."""
     nb = 1 << 20
-    rng = np.random.default_rng(4)
-    x = (rng.integers(0, 256, nb * 64) - 128).astype(np.float32)
+    x = blocks_values(nb, 4)
     batch = capi.Batch([(8, 8)] * nb)
     da, db = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
     da.upload(x)
@@ -413,19 +412,38 @@ def leg_codec(hip):
     return res
 
 
+def ragged_shapes(n=10000, seed=7):
+    """Shapes of the ragged mixed batch: each axis uniform over {8, ..., 1024}; returns the
+    shapes and the generator, positioned to draw the values."""
+    rng = np.random.default_rng(seed)
+    lengths = [8, 16, 32, 64, 128, 256, 512, 1024]
+    return list(zip(rng.choice(lengths, n).tolist(), rng.choice(lengths, n).tolist())), rng
+
+
+def ragged_values(shapes, offsets, elements, rng):
+    """Values of the ragged batch, back to back at `offsets`: uniform[-1, 1) times a
+    per-array scale 10^U[-3, 3]."""
+    x = np.empty(elements, np.float32)
+    for o, (h, w) in zip(offsets, shapes):
+        x[o:o + h * w] = rng.uniform(-1, 1, h * w) * 10 ** rng.uniform(-3, 3)
+    return x
+
+
+def blocks_values(nb=1 << 20, seed=4):
+    """nb block-contiguous 8x8 blocks, pixel values integers 0..255 minus 128."""
+    rng = np.random.default_rng(seed)
+    return (rng.integers(0, 256, nb * 64) - 128).astype(np.float32)
+
+
 def leg_ragged(hip, lib):
     """10,000 float32 arrays, each axis uniform over {8, ..., 1024}, values uniform[-1, 1)
     times a per-array scale 10^U[-3, 3], seed 7: the batch object (one persistent launch per
     direction, and the grouped per-shape kernels; `fwd` / `inv` report the automatic choice)
     vs one plan launch per array (the same FP32 butterfly kernels)."""
-    rng = np.random.default_rng(7)
-    lengths = [8, 16, 32, 64, 128, 256, 512, 1024]
     n = 10000
-    shapes = list(zip(rng.choice(lengths, n).tolist(), rng.choice(lengths, n).tolist()))
+    shapes, rng = ragged_shapes(n, 7)
     batch = capi.Batch(shapes)
-    x = np.empty(batch.elements, np.float32)
-    for o, (h, w) in zip(batch.offsets, shapes):
-        x[o:o + h * w] = rng.uniform(-1, 1, h * w) * 10 ** rng.uniform(-3, 3)
+    x = ragged_values(shapes, batch.offsets, batch.elements, rng)
     da, db = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
     da.upload(x)
     res = {'arrays': n, 'pixels': int(x.size), 'strategy_auto': {1: 'persistent', 2: 'grouped'}[batch.strategy]}

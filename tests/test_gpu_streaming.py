@@ -1,0 +1,162 @@
+"""GPU tests of the streaming kernel forms against the FP64 oracle.
+
+Calls whose input plus output exceed 256 MiB (the Infinity Cache) run non-temporal
+instantiations of the pass kernels and of the tiny batch kernel: the forms every
+large-batch throughput figure comes from.  Each test runs one such call at the size the
+benchmark uses (or just past the threshold), compares a sample of arrays per shape with
+the FP64 oracle (2x the reference CPU's error level, as in the 64-shape parity gate), and
+compares the same arrays bitwise with a call small enough to take the cached forms (the
+arithmetic is the same; only the load/store hints differ).  Contract: reference
+src/plan.c:126-163 applied per array.
+"""
+import numpy as np
+import pytest
+
+import bench
+import support as S
+from bridct_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+STREAM_MIN_BYTES = 256 << 20
+# 2x the reference CPU's worst max-relative error against FP64 over the 64 shapes
+# (forward 2.23e-7, inverse 2.14e-7, measured on the reference built from source).
+FWD_INV_TOL = 4.5e-7
+
+
+def device_call(plan, mode, x, batch, in_place=False):
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_out = d_in if in_place else capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    assert plan.apply_device(mode, d_in.ptr, d_out.ptr, batch) == 0
+    assert plan.synchronize() == 0
+    y = np.empty_like(x)
+    d_out.download(y)
+    d_in.free()
+    d_out.free()
+    return y
+
+
+@pytest.mark.parametrize('batch,in_place', [(40, False), (40, True), (128, False)])
+def test_1024_streaming_batch_against_oracle(batch, in_place):
+    """40 arrays (320 MiB in + out) and 128 arrays (the dense-route comparison's butterfly
+    leg, 1 GiB) of 1024^2: forward, inverse and fused round trip through the streaming
+    row and column passes."""
+    h = w = 1024
+    rng = np.random.default_rng(batch)
+    x = rng.uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    assert 2 * x.nbytes > STREAM_MIN_BYTES
+    sample = [0, batch // 2 + 1, batch - 1]
+    plan = capi.Plan(h, w)
+    try:
+        for mode in (0, 1, 2):
+            print('1024^2 x %d mode %d in_place %d' % (batch, mode, in_place), flush=True)
+            y = device_call(plan, mode, x, batch, in_place)
+            small = device_call(plan, mode, np.ascontiguousarray(x[sample]), len(sample))
+            assert np.array_equal(y[sample], small), 'streaming and cached forms differ'
+            for i in sample:
+                ref = S.oracle_2d(x[i], mode)
+                err = S.max_rel(y[i], ref)
+                assert err < (FWD_INV_TOL if mode < 2 else 1.5e-6), (mode, i, err)
+            assert np.isfinite(y).all()
+    finally:
+        plan.close()
+
+
+def test_ragged_10000_grouped_streaming_against_oracle():
+    """The ragged mix of 10,000 arrays (seed 7, 2.7 GB per buffer): the automatic strategy
+    (grouped, per-shape kernels in gather form, streaming hints) in one call; for every
+    distinct shape the first array against the FP64 oracle and, bitwise, against a plan of
+    that shape on that array alone; every array's round trip back to its input."""
+    shapes, rng = bench.ragged_shapes(10000, 7)
+    b = capi.Batch(shapes)
+    try:
+        assert b.strategy == capi.BATCH_GROUPED
+        x = bench.ragged_values(shapes, b.offsets, b.elements, rng)
+        assert 2 * x.nbytes > STREAM_MIN_BYTES
+        d_x, d_y, d_z = (capi.DeviceBuffer(x.nbytes) for _ in range(3))
+        d_x.upload(x)
+        print('ragged 10000 forward', flush=True)
+        assert b.apply_device(capi.FORWARD, d_x.ptr, d_y.ptr) == 0
+        print('ragged 10000 inverse', flush=True)
+        assert b.apply_device(capi.INVERSE, d_y.ptr, d_z.ptr) == 0
+        assert b.synchronize() == 0
+        y = np.empty_like(x)
+        z = np.empty_like(x)
+        d_y.download(y)
+        d_z.download(z)
+        for d in (d_x, d_y, d_z):
+            d.free()
+        first = {}
+        for i, s in enumerate(shapes):
+            first.setdefault(s, i)
+        assert len(first) == 64
+        plans = {}
+        worst = 0.0
+        for (h, w), i in sorted(first.items()):
+            o = b.offsets[i]
+            a = x[o:o + h * w].reshape(h, w)
+            got = y[o:o + h * w].reshape(h, w)
+            ref = S.oracle_2d(a, 0)
+            err = S.max_rel(got, ref)
+            worst = max(worst, err)
+            assert err < FWD_INV_TOL, ((h, w), err)
+            back = S.oracle_2d(got, 1)
+            assert S.max_rel(z[o:o + h * w].reshape(h, w), back) < FWD_INV_TOL, (h, w)
+            p = plans.setdefault((h, w), capi.Plan(h, w))
+            one = np.empty_like(a)
+            assert p.apply_batch(capi.FORWARD, np.ascontiguousarray(a), one) == 0
+            assert np.array_equal(one, got), ('grouped streaming result differs from the plan', h, w)
+        for p in plans.values():
+            p.close()
+        for o, (h, w) in zip(b.offsets, shapes):
+            a, r = x[o:o + h * w], z[o:o + h * w]
+            assert np.abs(r - a).max() <= 1.5e-6 * np.abs(a).max(), (h, w)
+        print('ragged worst forward max-rel vs FP64 %.3e' % worst, flush=True)
+    finally:
+        b.close()
+
+
+def test_blocks_8x8_streaming_against_oracle():
+    """1,048,576 8x8 blocks (256 MiB per buffer, pixel values 0..255 minus 128) through the
+    tiny batch kernel's streaming form: every block against the FP64 oracle (one GEMM with
+    the Kronecker product Q_8 (x) Q_8), and the first 65,536 blocks bitwise against a
+    cached-form call of those blocks alone."""
+    nb = 1 << 20
+    x = bench.blocks_values(nb, 4)
+    assert 2 * x.nbytes > STREAM_MIN_BYTES
+    q = S.qmatrix(8)
+    kron = np.kron(q, q)                     # vec(Q X Q^T) = (Q (x) Q) vec(X), row-major
+    sub = 65536
+    big = capi.Batch([(8, 8)] * nb)
+    small = capi.Batch([(8, 8)] * sub)
+    try:
+        for mode, mat in ((capi.FORWARD, kron), (capi.INVERSE, kron.T)):
+            print('8x8 blocks mode', mode, flush=True)
+            d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+            d_in.upload(x)
+            assert big.apply_device(mode, d_in.ptr, d_out.ptr) == 0
+            assert big.synchronize() == 0
+            y = np.empty_like(x)
+            d_out.download(y)
+            d_in.free()
+            d_out.free()
+            ref = x.reshape(nb, 64).astype(np.float64) @ mat.T
+            got = y.reshape(nb, 64)
+            diff = np.abs(got - ref).max(axis=1)
+            scale = np.abs(ref).max(axis=1)
+            assert (scale > 0).all()
+            assert float((diff / scale).max()) < FWD_INV_TOL, float((diff / scale).max())
+            part = np.ascontiguousarray(x[:sub * 64])
+            d_in, d_out = capi.DeviceBuffer(part.nbytes), capi.DeviceBuffer(part.nbytes)
+            d_in.upload(part)
+            assert small.apply_device(mode, d_in.ptr, d_out.ptr) == 0
+            assert small.synchronize() == 0
+            yp = np.empty_like(part)
+            d_out.download(yp)
+            d_in.free()
+            d_out.free()
+            assert np.array_equal(yp, y[:sub * 64]), 'streaming and cached tiny kernels differ'
+    finally:
+        big.close()
+        small.close()
