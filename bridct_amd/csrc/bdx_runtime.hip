@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <atomic>
+
 #include "bdx_kernels.h"
 #include "bdx_lines.h"
 #include "bridct_device.h"
@@ -46,9 +48,46 @@ __global__ void k_zero(unsigned *p, unsigned n)
         p[i] = 0u;
 }
 
+// Failure injection for the tests of the allocation-failure paths, and a count of the
+// runtime resources the library holds (device and pinned buffers, streams, events).
+// fault_after < 0: off; k >= 0: k more acquisitions succeed, the next one fails, then off.
+std::atomic<long> fault_after{-1};
+std::atomic<long> live_resources{0};
+
+// True when this acquisition must fail.
+bool injected_fault()
+{
+    long n = fault_after.load(std::memory_order_relaxed);
+    while (n >= 0)
+        if (fault_after.compare_exchange_weak(n, n - 1, std::memory_order_relaxed))
+            return n == 0;
+    return false;
+}
+
+int acquired(bool ok)
+{
+    if (!ok)
+        return -1;
+    live_resources.fetch_add(1, std::memory_order_relaxed);
+    return 0;
+}
+
+int released(void *handle, bool ok)
+{
+    if (handle && ok)
+        live_resources.fetch_sub(1, std::memory_order_relaxed);
+    return ok ? 0 : -1;
+}
+
 }  // namespace
 
 extern "C" {
+
+long bdx_debug_alloc_faults(long after)
+{
+    fault_after.store(after < 0 ? -1 : after, std::memory_order_relaxed);
+    return live_resources.load(std::memory_order_relaxed);
+}
 
 int bdx_device_ready(void)
 {
@@ -66,28 +105,28 @@ int bdx_device_ready(void)
 
 int bdx_malloc(void **ptr, size_t bytes)
 {
-    return hipMalloc(ptr, bytes) == hipSuccess ? 0 : -1;
+    return acquired(!injected_fault() && hipMalloc(ptr, bytes) == hipSuccess);
 }
 
 int bdx_free(void *ptr)
 {
-    return hipFree(ptr) == hipSuccess ? 0 : -1;
+    return released(ptr, hipFree(ptr) == hipSuccess);
 }
 
 int bdx_host_malloc(void **ptr, size_t bytes)
 {
-    return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? 0 : -1;
+    return acquired(!injected_fault() && hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess);
 }
 
 int bdx_host_free(void *ptr)
 {
-    return hipHostFree(ptr) == hipSuccess ? 0 : -1;
+    return released(ptr, hipHostFree(ptr) == hipSuccess);
 }
 
 int bdx_event_create(void **event)
 {
     hipEvent_t e;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+    if (acquired(!injected_fault() && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess))
         return -1;
     *event = (void *)e;
     return 0;
@@ -95,7 +134,7 @@ int bdx_event_create(void **event)
 
 int bdx_event_destroy(void *event)
 {
-    return hipEventDestroy((hipEvent_t)event) == hipSuccess ? 0 : -1;
+    return released(event, hipEventDestroy((hipEvent_t)event) == hipSuccess);
 }
 
 int bdx_event_record(void *event, void *stream)
@@ -126,7 +165,7 @@ int bdx_stream_wait_event(void *stream, void *event)
 int bdx_stream_create(void **stream)
 {
     hipStream_t s;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    if (acquired(!injected_fault() && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess))
         return -1;
     *stream = (void *)s;
     return 0;
@@ -134,7 +173,7 @@ int bdx_stream_create(void **stream)
 
 int bdx_stream_destroy(void *stream)
 {
-    return hipStreamDestroy((hipStream_t)stream) == hipSuccess ? 0 : -1;
+    return released(stream, hipStreamDestroy((hipStream_t)stream) == hipSuccess);
 }
 
 int bdx_stream_sync(void *stream)

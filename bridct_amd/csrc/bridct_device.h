@@ -127,6 +127,8 @@ int bdx_graph_destroy(void *graph_exec);
 #define BDX_ROUNDTRIP 2
 
 int bdx_device_ready(void);
+/* Failure injection and resource count (bridct_debug_alloc_faults in bridct_gpu.h). */
+long bdx_debug_alloc_faults(long after);
 int bdx_malloc(void **ptr, size_t bytes);
 int bdx_free(void *ptr);
 int bdx_host_malloc(void **ptr, size_t bytes);   /* pinned host memory */

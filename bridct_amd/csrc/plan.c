@@ -529,6 +529,54 @@ static int overlaps(const void *a, size_t bytes_a, const void *b, size_t bytes_b
     return x <= y ? y - x < bytes_a : x - y < bytes_b;
 }
 
+/* GPU state of the legacy square API: one plan per size and calling thread, created by the
+ * thread's first call of that size and destroyed when the thread exits (a thread that is
+ * still running at process exit keeps its plans until then).  Per thread, so concurrent
+ * callers never share a plan, as with the reference's caller-owned workspaces; the plan's
+ * tables and staging buffers are built once instead of on every call. */
+static pthread_key_t legacy_key;
+static pthread_once_t legacy_once = PTHREAD_ONCE_INIT;
+static int legacy_key_ok;
+
+typedef struct {
+    bridct_plan *plan[6];   /* n = 8 << i */
+} legacy_plans;
+
+static void legacy_release(void *arg)
+{
+    legacy_plans *l = arg;
+    for (int i = 0; i < 6; i++)
+        bridct_plan_destroy(l->plan[i]);
+    free(l);
+}
+
+static void legacy_key_create(void)
+{
+    legacy_key_ok = pthread_key_create(&legacy_key, legacy_release) == 0;
+}
+
+static bridct_plan *legacy_plan(int n)
+{
+    pthread_once(&legacy_once, legacy_key_create);
+    if (!legacy_key_ok)
+        return NULL;
+    legacy_plans *l = pthread_getspecific(legacy_key);
+    if (!l) {
+        if (!(l = calloc(1, sizeof(*l))))
+            return NULL;
+        if (pthread_setspecific(legacy_key, l)) {
+            free(l);
+            return NULL;
+        }
+    }
+    int i = 0;
+    while ((8 << i) < n)
+        i++;
+    if (!l->plan[i])
+        l->plan[i] = bridct_plan_create(n, n);
+    return l->plan[i];
+}
+
 /* src/bridct.c:30-45: legacy square transform with caller workspace.  The checks and
  * their precedence are the reference's; the workspace itself is not needed on the GPU. */
 int bridct_apply(int n, enum bridct_mode mode, const float *input, float *output,
@@ -546,17 +594,20 @@ int bridct_apply(int n, enum bridct_mode mode, const float *input, float *output
         overlaps(input, bytes, workspace, needed * sizeof(float)) ||
         overlaps(output, bytes, workspace, needed * sizeof(float)))
         return BRIDCT_OVERLAPPING_BUFFERS;
-    bridct_plan *plan = bridct_plan_create(n, n);
+    bridct_plan *plan = legacy_plan(n);
     if (!plan)
         return BRIDCT_DEVICE_ERROR;
-    const int status = run_host(plan, (int)mode, input, output, 1);
-    bridct_plan_destroy(plan);
-    return status;
+    return run_host(plan, (int)mode, input, output, 1);
 }
 
 int bridct_gpu_available(void)
 {
     return bdx_device_ready() == 0;
+}
+
+long bridct_debug_alloc_faults(long after)
+{
+    return bdx_debug_alloc_faults(after);
 }
 
 void *bridct_gpu_malloc(size_t bytes)

@@ -1,32 +1,45 @@
-/* bridct_numpy: CPython/NumPy interface to the MI355X BRiDCT plan library.
+/* bridct_numpy -- the NumPy face of the MI355X BRiDCT library (no PyTorch, no CPU path).
  *
- * Same module, type and contract as the reference extension (python/bridct_numpy.c):
- *   Plan(height, width); forward / inverse / roundtrip(input, *, out=None); close();
- *   getters shape, route, closed; module constant core_version.
- * Inputs are exact float32 ndarrays of shape (h, w) or (B, h, w), native byte order,
- * C-contiguous and aligned; `out` must also be writable, of the same ndim and size, and
- * must not overlap the input.  TypeError for type/keyword problems, ValueError for
- * shape/layout/overlap, RuntimeError for closed/busy/re-initialized plans and failed
- * calls, MemoryError for allocation.  roundtrip materializes the normalized FP32
- * spectrum (python/bridct_numpy.c:114-133), so it equals inverse(forward(x)) bitwise.
- * The GIL is released for inputs of 16384 elements or more, and a per-plan busy flag
- * rejects concurrent use (python/bridct_numpy.c:142-147, 191-198).
+ * Drop-in contract (the reference module of the same name, described by its
+ * python/README.md and checked by python/test_api.py and python/test_hardening.py):
  *
- * The whole batch goes to the GPU in one call (bridct_plan_apply_batch), not one call
- * per image as in the reference loop (python/bridct_numpy.c:114-133).
+ *   Plan(height, width)                 both axes powers of two in 8..1024, else ValueError
+ *     .forward(a, *, out=None)          orthonormal 2D DCT-II over the last two axes
+ *     .inverse(a, *, out=None)          orthonormal 2D DCT-III
+ *     .roundtrip(a, *, out=None)        forward, FP32 spectrum materialized, then inverse;
+ *                                       equal bitwise to inverse(forward(a))
+ *     .close()                          idempotent
+ *     .shape / .route / .closed
+ *   core_version                        the C library's BRIDCT_VERSION
  *
- * RaggedPlan(shapes) has no counterpart in the reference: a list of arrays of mixed
- * supported shapes goes through one persistent GPU launch per direction (bridct_batch_*).
- * forward / inverse / roundtrip(arrays, *, out=None) take a list or tuple of float32
- * (h_i, w_i) arrays, validated like Plan inputs, and return a list (or fill `out`, a list
- * of writable arrays of the same shapes, each disjoint from its input).
+ *   a: an exact ndarray (no subclass, no conversion) of native float32, C order, float
+ *      aligned, shape (height, width) or (B, height, width) with B >= 1; read-only is fine
+ *      and it is never modified.  out: a writable ndarray of exactly a's shape under the
+ *      same rules, sharing no byte with a; returned as is.  Without out every call
+ *      returns a new owning array.
+ *   Exceptions: TypeError (argument kinds, dtype, byte order, keywords), ValueError
+ *   (shape, layout, alignment, writability, overlap), RuntimeError (object not open, in
+ *   use by a nested call or another thread, re-initialization, a failed GPU call),
+ *   MemoryError (allocation).
+ *   Calls of 16384 elements or more run without the GIL.  An object is used by one call
+ *   at a time: a second caller -- another thread, or Python code run from inside the call
+ *   (an __index__ of a constructor argument, a NumPy allocator hook) -- gets RuntimeError
+ *   and the first call is unaffected.
+ *   The module refuses to load in any interpreter but the main one (ImportError), and
+ *   refuses to build for a free-threaded CPython.
  *
- * Codec8(quant) has no counterpart in the reference either: the 8x8 block codec step
- * (bridct_codec8_*).  quant: float32 ndarray of 64 finite positive steps (shape (64,) or
- * (8, 8), row-major u * 8 + v).  encode(image, *, out=None): uint8 (H, W) image, H and W
- * multiples of 8, rows may be strided -> int16 (H/8, W/8, 8, 8) quantized coefficients;
- * decode(coeffs, *, out=None): the reverse (out may have strided rows).  Same exception
- * split, busy flag and GIL rule as Plan.
+ * Beyond the reference:
+ *   RaggedPlan(shapes)                  a list of arrays of mixed supported shapes, one
+ *                                       batch launch per direction (bridct_batch_*);
+ *                                       methods take and return lists of 2D arrays.
+ *   Codec8(quant)                       the 8x8 block codec step (bridct_codec8_*):
+ *                                       encode uint8 (H, W) -> int16 (H/8, W/8, 8, 8)
+ *                                       quantized coefficients, decode the reverse.
+ *
+ * Every object goes through one lifecycle (struct life below): NEW (allocated, not
+ * initialized) -> SETUP (inside __init__) -> OPEN -> BUSY (inside a call) -> OPEN ...
+ * -> SHUT (closed; never opens again).  A failed __init__ returns to NEW, so it may be
+ * retried; close() of a NEW object is a no-op.
  */
 #define PY_SSIZE_T_CLEAN
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
@@ -42,252 +55,421 @@
 #include <string.h>
 
 #ifdef Py_GIL_DISABLED
-#error "bridct_numpy needs a CPython build with the GIL."
+#error "bridct_numpy relies on the GIL to guard each object's state; build it for a CPython with the GIL."
 #endif
 
-typedef struct {
-    PyObject_HEAD
-    bridct_plan *plan;
-    size_t elements;
-    int height, width, initialized, busy;
-} Plan;
+/* Element count from which a call releases the GIL while the GPU works. */
+#define NOGIL_ELEMENTS 16384
 
-static int Plan_init(Plan *self, PyObject *args, PyObject *kwargs)
+/* ---- lifecycle ---------------------------------------------------------------------------- */
+
+enum { LIFE_NEW, LIFE_SETUP, LIFE_OPEN, LIFE_BUSY, LIFE_SHUT };
+
+/* Every state change happens with the GIL held, so test-and-set is atomic across threads. */
+typedef struct {
+    int state;
+    const char *kind;   /* type name for messages */
+} life;
+
+static int life_in_use(const life *l)
 {
-    static char *keywords[] = {"height", "width", NULL};
-    int height, width;
-    if (self->initialized) {
-        PyErr_SetString(PyExc_RuntimeError, "A Plan cannot be reinitialized.");
-        return -1;
-    }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This Plan is already initializing or executing.");
-        return -1;
-    }
-    self->busy = 1;
-    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "ii:Plan", keywords, &height, &width))
-        goto fail;
-    if (!bridct_supported(height, width)) {
-        PyErr_SetString(PyExc_ValueError, "Both dimensions must be powers of two from 8 through 1024.");
-        goto fail;
-    }
-    if (!bridct_gpu_available()) {
-        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT GPU plan cannot be created.");
-        goto fail;
-    }
-    bridct_plan *plan = bridct_plan_create(height, width);
-    if (!plan) {
-        PyErr_NoMemory();
-        goto fail;
-    }
-    self->plan = plan;
-    self->elements = (size_t)height * (size_t)width;
-    self->height = height;
-    self->width = width;
-    self->initialized = 1;
-    self->busy = 0;
-    return 0;
-fail:
-    self->busy = 0;
+    PyErr_Format(PyExc_RuntimeError,
+                 "This %s is in use by another call (another thread, or Python code running inside "
+                 "the call); use one %s per thread.",
+                 l->kind, l->kind);
     return -1;
 }
 
-static void Plan_dealloc(Plan *self)
+/* Begins __init__: only a NEW object may be initialized. */
+static int life_begin_setup(life *l)
 {
-    bridct_plan_destroy(self->plan);
-    Py_TYPE(self)->tp_free((PyObject *)self);
+    switch (l->state) {
+    case LIFE_NEW:
+        l->state = LIFE_SETUP;
+        return 0;
+    case LIFE_SETUP:
+    case LIFE_BUSY:
+        return life_in_use(l);
+    default:
+        PyErr_Format(PyExc_RuntimeError, "A %s is initialized once; create a new one instead.", l->kind);
+        return -1;
+    }
 }
 
-static PyObject *Plan_close(Plan *self, PyObject *unused)
+static void life_end_setup(life *l, int ok)
 {
-    (void)unused;
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError,
-                        "This Plan is already initializing or executing, possibly in another thread.");
-        return NULL;
-    }
-    bridct_plan_destroy(self->plan);
-    self->plan = NULL;
-    Py_RETURN_NONE;
+    l->state = ok ? LIFE_OPEN : LIFE_NEW;
 }
 
-static PyArrayObject *validated(PyObject *object, const Plan *self, int writable)
+/* Begins a transform call: only an OPEN object may run one. */
+static int life_begin_call(life *l)
 {
-    if (!PyArray_CheckExact(object)) {
-        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+    switch (l->state) {
+    case LIFE_OPEN:
+        l->state = LIFE_BUSY;
+        return 0;
+    case LIFE_SETUP:
+    case LIFE_BUSY:
+        return life_in_use(l);
+    case LIFE_NEW:
+        PyErr_Format(PyExc_RuntimeError, "This %s was never initialized.", l->kind);
+        return -1;
+    default:
+        PyErr_Format(PyExc_RuntimeError, "This %s is closed.", l->kind);
+        return -1;
+    }
+}
+
+static void life_end_call(life *l)
+{
+    l->state = LIFE_OPEN;
+}
+
+/* close(): 1 when the caller must release the native object now, 0 when there is nothing
+ * to release, -1 (error set) while a call or an initialization is running. */
+static int life_close(life *l)
+{
+    switch (l->state) {
+    case LIFE_SETUP:
+    case LIFE_BUSY:
+        return life_in_use(l);
+    case LIFE_OPEN:
+        l->state = LIFE_SHUT;
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+static int life_open(const life *l)
+{
+    return l->state == LIFE_OPEN || l->state == LIFE_BUSY;
+}
+
+/* ---- argument helpers ------------------------------------------------------------------------ */
+
+/* method(input, *, out=None) in vectorcall form. */
+static int parse_input_out(PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames, PyObject **input,
+                           PyObject **out)
+{
+    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
+    if (nargs != 1) {
+        PyErr_SetString(PyExc_TypeError, "Expected exactly one positional argument; pass out= by keyword.");
+        return -1;
+    }
+    *input = args[0];
+    *out = Py_None;
+    for (Py_ssize_t k = 0; k < nkw; k++) {
+        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, k), "out") != 0 || nkw > 1) {
+            PyErr_SetString(PyExc_TypeError, "out is the only keyword argument.");
+            return -1;
+        }
+        *out = args[nargs + k];
+    }
+    return 0;
+}
+
+/* Byte range [lo, hi) spanned by an array (strides may be of either sign). */
+static void byte_span(PyArrayObject *a, uintptr_t *lo, uintptr_t *hi)
+{
+    intptr_t first = 0, last = 0;
+    for (int d = 0; d < PyArray_NDIM(a); d++) {
+        const intptr_t reach = (intptr_t)(PyArray_DIM(a, d) - 1) * (intptr_t)PyArray_STRIDE(a, d);
+        if (reach < 0)
+            first += reach;
+        else
+            last += reach;
+    }
+    *lo = (uintptr_t)PyArray_DATA(a) + first;
+    *hi = (uintptr_t)PyArray_DATA(a) + last + (uintptr_t)PyArray_ITEMSIZE(a);
+}
+
+static int spans_meet(PyArrayObject *a, PyArrayObject *b)
+{
+    if (PyArray_SIZE(a) == 0 || PyArray_SIZE(b) == 0)
+        return 0;
+    uintptr_t a0, a1, b0, b1;
+    byte_span(a, &a0, &a1);
+    byte_span(b, &b0, &b1);
+    return a0 < b1 && b0 < a1;
+}
+
+/* The kind checks shared by every array argument: exact ndarray of `type` in native byte
+ * order.  Returns the array (borrowed) or NULL with TypeError. */
+static PyArrayObject *exact_array(PyObject *o, int type, const char *what)
+{
+    if (!PyArray_CheckExact(o)) {
+        PyErr_Format(PyExc_TypeError, "%s must be an exact numpy.ndarray (no subclass, no conversion).", what);
         return NULL;
     }
-    PyArrayObject *a = (PyArrayObject *)object;
-    if (PyArray_TYPE(a) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(a)) {
-        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32.");
-        return NULL;
-    }
-    const int nd = PyArray_NDIM(a);
-    if ((nd != 2 && nd != 3) || PyArray_DIM(a, nd - 2) != self->height ||
-        PyArray_DIM(a, nd - 1) != self->width || (nd == 3 && PyArray_DIM(a, 0) < 1)) {
-        PyErr_SetString(PyExc_ValueError, "Expected (height, width) or (batch, height, width), with batch >= 1.");
-        return NULL;
-    }
-    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
-        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, float-aligned storage.");
-        return NULL;
-    }
-    if (writable && !PyArray_ISWRITEABLE(a)) {
-        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
+    PyArrayObject *a = (PyArrayObject *)o;
+    if (PyArray_TYPE(a) != type || !PyArray_ISNOTSWAPPED(a)) {
+        PyErr_Format(PyExc_TypeError, "%s must have native-endian dtype %s.", what,
+                     type == NPY_FLOAT32 ? "float32" : type == NPY_INT16 ? "int16" : "uint8");
         return NULL;
     }
     return a;
 }
 
-static PyObject *execute(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
-                         int mode)
+/* Layout rules shared by float32 inputs and outputs: C order, aligned, writable if asked. */
+static int dense_layout(PyArrayObject *a, int writable, const char *what)
 {
-    if (!self->plan) {
-        PyErr_SetString(PyExc_RuntimeError, "This Plan is closed or uninitialized.");
+    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
+        PyErr_Format(PyExc_ValueError, "%s must be C-contiguous and aligned.", what);
+        return -1;
+    }
+    if (writable && !PyArray_ISWRITEABLE(a)) {
+        PyErr_Format(PyExc_ValueError, "%s must be writable.", what);
+        return -1;
+    }
+    return 0;
+}
+
+/* Validates an array argument of one object (`ctx`); `is_out` selects the output rules. */
+typedef PyArrayObject *(*array_check)(PyObject *o, int is_out, const void *ctx);
+
+/* The output of a call: a new array of shape `dims` and `type`, or the caller's `out` after
+ * `check` and the shape and overlap rules.  Returns a new reference or NULL. */
+static PyArrayObject *output_for(PyObject *out, PyArrayObject *in, int nd, const npy_intp *dims, int type,
+                                 array_check check, const void *ctx)
+{
+    if (out == Py_None)
+        return (PyArrayObject *)PyArray_SimpleNew(nd, (npy_intp *)dims, type);
+    PyArrayObject *o = check(out, 1, ctx);
+    if (!o)
+        return NULL;
+    if (PyArray_NDIM(o) != nd || memcmp(PyArray_DIMS(o), dims, (size_t)nd * sizeof(npy_intp)) != 0) {
+        PyErr_SetString(PyExc_ValueError, "out must have exactly the shape of the result.");
         return NULL;
     }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This Plan is executing in another thread; use one Plan per thread.");
+    if (spans_meet(in, o)) {
+        PyErr_SetString(PyExc_ValueError, "out must not share memory with the input.");
         return NULL;
     }
-    /* Array allocation can run Python code; the flag covers setup as well as compute. */
-    self->busy = 1;
-    PyArrayObject *out = NULL;
-    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
-    if (nargs != 1 || nkw > 1) {
-        PyErr_SetString(PyExc_TypeError, "Use method(input, *, out=None).");
-        goto fail;
-    }
-    PyObject *out_obj = Py_None;
-    if (nkw) {
-        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
-            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
-            goto fail;
-        }
-        out_obj = args[1];
-    }
-    PyArrayObject *in = validated(args[0], self, 0);
-    if (!in)
-        goto fail;
-    if (out_obj == Py_None) {
-        out = (PyArrayObject *)PyArray_SimpleNew(PyArray_NDIM(in), PyArray_DIMS(in), NPY_FLOAT32);
-        if (!out)
-            goto fail;
-    } else {
-        PyArrayObject *given = validated(out_obj, self, 1);
-        if (!given)
-            goto fail;
-        if (PyArray_NDIM(in) != PyArray_NDIM(given) || PyArray_SIZE(in) != PyArray_SIZE(given)) {
-            PyErr_SetString(PyExc_ValueError, "Input and output shapes must match exactly.");
-            goto fail;
-        }
-        const uintptr_t a = (uintptr_t)PyArray_DATA(in), b = (uintptr_t)PyArray_DATA(given);
-        if ((a <= b ? b - a : a - b) < (size_t)PyArray_NBYTES(in)) {
-            PyErr_SetString(PyExc_ValueError, "Input and output must not overlap.");
-            goto fail;
-        }
-        out = given;
-        Py_INCREF(out);
-    }
-    const float *x = PyArray_DATA(in);
-    float *y = PyArray_DATA(out);
-    const size_t batch = PyArray_NDIM(in) == 3 ? (size_t)PyArray_DIM(in, 0) : 1;
-    int status;
-    if (PyArray_SIZE(in) >= 16384) {
-        Py_BEGIN_ALLOW_THREADS
-        status = bridct_plan_apply_batch(self->plan, mode, x, y, batch);
-        Py_END_ALLOW_THREADS
-    } else {
-        status = bridct_plan_apply_batch(self->plan, mode, x, y, batch);
-    }
-    if (status) {
-        PyErr_Format(PyExc_RuntimeError, "BRiDCT failed with status %d.", status);
-        goto fail;
-    }
-    self->busy = 0;
-    return (PyObject *)out;
-fail:
-    self->busy = 0;
-    Py_XDECREF(out);
+    Py_INCREF(o);
+    return o;
+}
+
+static PyObject *gpu_status_error(const char *what, int status)
+{
+    PyErr_Format(PyExc_RuntimeError, "%s failed on the GPU with status %d.", what, status);
     return NULL;
 }
 
-static PyObject *Plan_forward(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+/* ---- Plan ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    PyObject_HEAD
+    life life;
+    bridct_plan *plan;
+    int height, width;
+} PlanObject;
+
+static PyObject *Plan_new(PyTypeObject *type, PyObject *args, PyObject *kwargs)
 {
-    return execute(self, args, nargs, kw, BRIDCT_FORWARD);
+    (void)args;
+    (void)kwargs;
+    PlanObject *self = (PlanObject *)type->tp_alloc(type, 0);
+    if (self)
+        self->life = (life){LIFE_NEW, "Plan"};
+    return (PyObject *)self;
 }
 
-static PyObject *Plan_inverse(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static int Plan_init(PlanObject *self, PyObject *args, PyObject *kwargs)
 {
-    return execute(self, args, nargs, kw, BRIDCT_INVERSE);
+    static char *names[] = {"height", "width", NULL};
+    int h = 0, w = 0;
+    if (life_begin_setup(&self->life))
+        return -1;
+    /* Parsing can run Python code (__index__); the object is already in SETUP then. */
+    int ok = PyArg_ParseTupleAndKeywords(args, kwargs, "ii:Plan", names, &h, &w);
+    if (ok && !bridct_supported(h, w)) {
+        PyErr_Format(PyExc_ValueError, "Unsupported shape (%d, %d): each axis must be a power of two "
+                     "from 8 through 1024.", h, w);
+        ok = 0;
+    }
+    if (ok && !bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU; this library has no CPU path.");
+        ok = 0;
+    }
+    bridct_plan *plan = NULL;
+    if (ok && !(plan = bridct_plan_create(h, w))) {
+        PyErr_SetString(PyExc_MemoryError, "Could not allocate the GPU plan.");
+        ok = 0;
+    }
+    if (ok) {
+        self->plan = plan;
+        self->height = h;
+        self->width = w;
+    }
+    life_end_setup(&self->life, ok);
+    return ok ? 0 : -1;
 }
 
-static PyObject *Plan_roundtrip(Plan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static void Plan_dealloc(PlanObject *self)
 {
-    return execute(self, args, nargs, kw, BRIDCT_GPU_FORWARD_THEN_INVERSE);
+    bridct_plan_destroy(self->plan);
+    Py_TYPE(self)->tp_free((PyObject *)self);
 }
 
-static PyObject *Plan_shape(Plan *self, void *unused)
+static PyObject *Plan_close(PlanObject *self, PyObject *unused)
 {
     (void)unused;
+    const int r = life_close(&self->life);
+    if (r < 0)
+        return NULL;
+    if (r) {
+        bridct_plan_destroy(self->plan);
+        self->plan = NULL;
+    }
+    Py_RETURN_NONE;
+}
+
+/* A Plan's input or output: (h, w) or (B, h, w) float32, B >= 1. */
+static PyArrayObject *plan_array(PyObject *o, int is_out, const void *ctx)
+{
+    const PlanObject *p = ctx;
+    const char *what = is_out ? "out" : "The input";
+    PyArrayObject *a = exact_array(o, NPY_FLOAT32, what);
+    if (!a)
+        return NULL;
+    const int nd = PyArray_NDIM(a);
+    if (nd < 2 || nd > 3 || PyArray_DIM(a, nd - 2) != p->height || PyArray_DIM(a, nd - 1) != p->width ||
+        (nd == 3 && PyArray_DIM(a, 0) < 1)) {
+        PyErr_Format(PyExc_ValueError, "%s must have shape (%d, %d) or (batch, %d, %d) with batch >= 1.", what,
+                     p->height, p->width, p->height, p->width);
+        return NULL;
+    }
+    return dense_layout(a, is_out, what) ? NULL : a;
+}
+
+static PyObject *plan_run(PlanObject *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames, int mode)
+{
+    if (life_begin_call(&self->life))
+        return NULL;
+    PyObject *input, *out_arg, *result = NULL;
+    PyArrayObject *in, *out = NULL;
+    if (parse_input_out(args, nargs, kwnames, &input, &out_arg))
+        goto done;
+    if (!(in = plan_array(input, 0, self)))
+        goto done;
+    /* May allocate through NumPy's data allocator, which can run Python code. */
+    out = output_for(out_arg, in, PyArray_NDIM(in), PyArray_DIMS(in), NPY_FLOAT32, plan_array, self);
+    if (!out)
+        goto done;
+    const size_t batch = PyArray_NDIM(in) == 3 ? (size_t)PyArray_DIM(in, 0) : 1;
+    const float *src = PyArray_DATA(in);
+    float *dst = PyArray_DATA(out);
+    int status;
+    if (PyArray_SIZE(in) >= NOGIL_ELEMENTS) {
+        Py_BEGIN_ALLOW_THREADS
+        status = bridct_plan_apply_batch(self->plan, mode, src, dst, batch);
+        Py_END_ALLOW_THREADS
+    } else {
+        status = bridct_plan_apply_batch(self->plan, mode, src, dst, batch);
+    }
+    if (status) {
+        gpu_status_error("The transform", status);
+        goto done;
+    }
+    result = (PyObject *)out;
+    out = NULL;
+done:
+    Py_XDECREF(out);
+    life_end_call(&self->life);
+    return result;
+}
+
+static PyObject *Plan_forward(PlanObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
+{
+    return plan_run(self, a, n, k, BRIDCT_FORWARD);
+}
+
+static PyObject *Plan_inverse(PlanObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
+{
+    return plan_run(self, a, n, k, BRIDCT_INVERSE);
+}
+
+static PyObject *Plan_roundtrip(PlanObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
+{
+    return plan_run(self, a, n, k, BRIDCT_GPU_FORWARD_THEN_INVERSE);
+}
+
+static PyObject *Plan_get_shape(PlanObject *self, void *unused)
+{
+    (void)unused;
+    if (self->life.state == LIFE_NEW || self->life.state == LIFE_SETUP)
+        Py_RETURN_NONE;
     return Py_BuildValue("(ii)", self->height, self->width);
 }
 
-static PyObject *Plan_route(Plan *self, void *unused)
+static PyObject *Plan_get_route(PlanObject *self, void *unused)
 {
     (void)unused;
-    if (!self->plan)
+    if (!life_open(&self->life))
         Py_RETURN_NONE;
     return PyUnicode_FromString(bridct_plan_route(self->plan));
 }
 
-static PyObject *Plan_closed(Plan *self, void *unused)
+static PyObject *Plan_get_closed(PlanObject *self, void *unused)
 {
     (void)unused;
-    return PyBool_FromLong(self->plan == NULL);
+    return PyBool_FromLong(!life_open(&self->life));
 }
 
-#define AS_METHOD(fn) (PyCFunction)(void (*)(void))(fn)
+#define VECTORCALL_METHOD(fn) (PyCFunction)(void (*)(void))(fn)
+
 static PyMethodDef Plan_methods[] = {
-    {"forward", AS_METHOD(Plan_forward), METH_FASTCALL | METH_KEYWORDS,
-     "forward(input, *, out=None): orthonormal 2D DCT-II over the last two axes, on the GPU."},
-    {"inverse", AS_METHOD(Plan_inverse), METH_FASTCALL | METH_KEYWORDS,
-     "inverse(input, *, out=None): orthonormal 2D DCT-III over the last two axes, on the GPU."},
-    {"roundtrip", AS_METHOD(Plan_roundtrip), METH_FASTCALL | METH_KEYWORDS,
-     "roundtrip(input, *, out=None): forward then inverse with a materialized spectrum."},
-    {"close", AS_METHOD(Plan_close), METH_NOARGS, "Release the GPU plan and its buffers."},
+    {"forward", VECTORCALL_METHOD(Plan_forward), METH_FASTCALL | METH_KEYWORDS,
+     "forward(input, *, out=None) -> orthonormal 2D DCT-II of the last two axes (GPU)."},
+    {"inverse", VECTORCALL_METHOD(Plan_inverse), METH_FASTCALL | METH_KEYWORDS,
+     "inverse(input, *, out=None) -> orthonormal 2D DCT-III of the last two axes (GPU)."},
+    {"roundtrip", VECTORCALL_METHOD(Plan_roundtrip), METH_FASTCALL | METH_KEYWORDS,
+     "roundtrip(input, *, out=None) -> inverse(forward(input)), spectrum kept in float32."},
+    {"close", VECTORCALL_METHOD(Plan_close), METH_NOARGS, "close() -> None; frees the GPU plan (idempotent)."},
     {NULL, NULL, 0, NULL}};
 
 static PyGetSetDef Plan_getset[] = {
-    {"shape", (getter)Plan_shape, NULL, "Planned (height, width).", NULL},
-    {"route", (getter)Plan_route, NULL, "Route category, or None when closed.", NULL},
-    {"closed", (getter)Plan_closed, NULL, "Whether the plan has been closed.", NULL},
+    {"shape", (getter)Plan_get_shape, NULL, "(height, width) of the plan; None before initialization.", NULL},
+    {"route", (getter)Plan_get_route, NULL, "Shape category string of the plan; None unless open.", NULL},
+    {"closed", (getter)Plan_get_closed, NULL, "True unless the plan is initialized and not yet closed.", NULL},
     {NULL, NULL, NULL, NULL, NULL}};
 
-static PyTypeObject PlanType = {
+static PyTypeObject Plan_Type = {
     PyVarObject_HEAD_INIT(NULL, 0)
     .tp_name = "bridct_numpy.Plan",
-    .tp_basicsize = sizeof(Plan),
-    .tp_dealloc = (destructor)Plan_dealloc,
+    .tp_doc = "Plan(height, width): reusable orthonormal float32 2D DCT on the GPU; one per thread.",
+    .tp_basicsize = sizeof(PlanObject),
     .tp_flags = Py_TPFLAGS_DEFAULT,
-    .tp_doc = "Reusable orthonormal float32 2D DCT plan on the GPU; one plan per concurrent worker.",
-    .tp_methods = Plan_methods,
-    .tp_getset = Plan_getset,
+    .tp_new = Plan_new,
     .tp_init = (initproc)Plan_init,
-    .tp_new = PyType_GenericNew};
+    .tp_dealloc = (destructor)Plan_dealloc,
+    .tp_methods = Plan_methods,
+    .tp_getset = Plan_getset};
 
-/* ---- RaggedPlan ------------------------------------------------------------------ */
+/* ---- RaggedPlan ------------------------------------------------------------------------------ */
 
 typedef struct {
     PyObject_HEAD
+    life life;
     bridct_batch *batch;
-    PyObject *shapes;           /* tuple of (h, w) tuples */
+    PyObject *shapes;             /* tuple of (h, w) tuples, kept after close */
+    int *heights, *widths;
     size_t count, elements;
-    float *stage_in, *stage_out;   /* host staging, allocated on first use */
-    int initialized, busy;
-} RaggedPlan;
+    float *stage_in, *stage_out;  /* host staging buffers, made on the first call */
+} RaggedObject;
 
-static void ragged_release(RaggedPlan *self)
+static PyObject *Ragged_new(PyTypeObject *type, PyObject *args, PyObject *kwargs)
+{
+    (void)args;
+    (void)kwargs;
+    RaggedObject *self = (RaggedObject *)type->tp_alloc(type, 0);
+    if (self)
+        self->life = (life){LIFE_NEW, "RaggedPlan"};
+    return (PyObject *)self;
+}
+
+static void ragged_free_native(RaggedObject *self)
 {
     bridct_batch_destroy(self->batch);
     self->batch = NULL;
@@ -296,199 +478,177 @@ static void ragged_release(RaggedPlan *self)
     self->stage_in = self->stage_out = NULL;
 }
 
-static int RaggedPlan_init(RaggedPlan *self, PyObject *args, PyObject *kwargs)
+/* Reads the shape list into self->heights / widths / shapes (new objects on success). */
+static int ragged_read_shapes(RaggedObject *self, PyObject *spec)
 {
-    static char *keywords[] = {"shapes", NULL};
-    PyObject *seq = NULL, *shapes = NULL;
-    int *hs = NULL, *ws = NULL;
-    if (self->initialized) {
-        PyErr_SetString(PyExc_RuntimeError, "A RaggedPlan cannot be reinitialized.");
-        return -1;
-    }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is already initializing or executing.");
-        return -1;
-    }
-    self->busy = 1;
-    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "O:RaggedPlan", keywords, &seq))
-        goto fail;
-    seq = PySequence_Fast(seq, "shapes must be a sequence of (height, width) pairs.");
+    PyObject *seq = PySequence_Fast(spec, "shapes must be a sequence of (height, width) pairs.");
     if (!seq)
-        goto fail;
+        return -1;
     const Py_ssize_t n = PySequence_Fast_GET_SIZE(seq);
+    PyObject *tuple = NULL;
+    int *hs = NULL, *ws = NULL;
     if (n < 1) {
         PyErr_SetString(PyExc_ValueError, "A RaggedPlan needs at least one shape.");
         goto fail;
     }
-    hs = malloc((size_t)n * sizeof(int));
-    ws = malloc((size_t)n * sizeof(int));
-    shapes = PyTuple_New(n);
-    if (!hs || !ws || !shapes) {
-        PyErr_NoMemory();
+    hs = PyMem_Malloc((size_t)n * sizeof(int));
+    ws = PyMem_Malloc((size_t)n * sizeof(int));
+    if (!hs || !ws || !(tuple = PyTuple_New(n))) {
+        if (!PyErr_Occurred())
+            PyErr_NoMemory();
         goto fail;
     }
     for (Py_ssize_t i = 0; i < n; i++) {
-        int h, w;
         PyObject *item = PySequence_Fast_GET_ITEM(seq, i);
         if (!PyTuple_Check(item) && !PyList_Check(item)) {
-            PyErr_SetString(PyExc_TypeError, "Each shape must be a (height, width) pair.");
+            PyErr_Format(PyExc_TypeError, "shapes[%zd] must be a (height, width) tuple or list.", i);
             goto fail;
         }
-        PyObject *pair = PySequence_Tuple(item);
+        if (PySequence_Size(item) != 2) {
+            PyErr_Format(PyExc_TypeError, "shapes[%zd] must hold exactly two integers.", i);
+            goto fail;
+        }
+        int dims[2];
+        for (int d = 0; d < 2; d++) {
+            PyObject *v = PySequence_GetItem(item, d);
+            if (!v)
+                goto fail;
+            const long x = PyLong_AsLong(v);
+            Py_DECREF(v);
+            if (x == -1 && PyErr_Occurred())
+                goto fail;
+            dims[d] = x < INT32_MIN || x > INT32_MAX ? 0 : (int)x;
+        }
+        if (!bridct_supported(dims[0], dims[1])) {
+            PyErr_Format(PyExc_ValueError, "shapes[%zd]: each axis must be a power of two from 8 through 1024.", i);
+            goto fail;
+        }
+        hs[i] = dims[0];
+        ws[i] = dims[1];
+        PyObject *pair = Py_BuildValue("(ii)", dims[0], dims[1]);
         if (!pair)
             goto fail;
-        const int ok = PyArg_ParseTuple(pair, "ii", &h, &w);
-        Py_DECREF(pair);
-        if (!ok)
-            goto fail;
-        if (!bridct_supported(h, w)) {
-            PyErr_SetString(PyExc_ValueError, "Both dimensions must be powers of two from 8 through 1024.");
-            goto fail;
-        }
-        hs[i] = h;
-        ws[i] = w;
-        PyObject *t = Py_BuildValue("(ii)", h, w);
-        if (!t)
-            goto fail;
-        PyTuple_SET_ITEM(shapes, i, t);
+        PyTuple_SET_ITEM(tuple, i, pair);
     }
-    if (!bridct_gpu_available()) {
-        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT GPU batch cannot be created.");
-        goto fail;
-    }
-    self->batch = bridct_batch_create((size_t)n, hs, ws);
-    if (!self->batch) {
-        PyErr_NoMemory();
-        goto fail;
-    }
-    self->count = (size_t)n;
-    self->elements = bridct_batch_elements(self->batch);
-    self->shapes = shapes;
-    self->initialized = 1;
-    self->busy = 0;
-    free(hs);
-    free(ws);
     Py_DECREF(seq);
+    self->heights = hs;
+    self->widths = ws;
+    self->shapes = tuple;
+    self->count = (size_t)n;
     return 0;
 fail:
-    self->busy = 0;
-    free(hs);
-    free(ws);
-    Py_XDECREF(shapes);
-    Py_XDECREF(seq);
+    PyMem_Free(hs);
+    PyMem_Free(ws);
+    Py_XDECREF(tuple);
+    Py_DECREF(seq);
     return -1;
 }
 
-static void RaggedPlan_dealloc(RaggedPlan *self)
+static int Ragged_init(RaggedObject *self, PyObject *args, PyObject *kwargs)
 {
-    ragged_release(self);
+    static char *names[] = {"shapes", NULL};
+    PyObject *spec = NULL;
+    if (life_begin_setup(&self->life))
+        return -1;
+    int ok = PyArg_ParseTupleAndKeywords(args, kwargs, "O:RaggedPlan", names, &spec) &&
+             ragged_read_shapes(self, spec) == 0;
+    if (ok && !bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU; this library has no CPU path.");
+        ok = 0;
+    }
+    if (ok && !(self->batch = bridct_batch_create(self->count, self->heights, self->widths))) {
+        PyErr_SetString(PyExc_MemoryError, "Could not allocate the GPU batch.");
+        ok = 0;
+    }
+    if (ok) {
+        self->elements = bridct_batch_elements(self->batch);
+    } else {
+        PyMem_Free(self->heights);
+        PyMem_Free(self->widths);
+        self->heights = self->widths = NULL;
+        Py_CLEAR(self->shapes);
+        self->count = 0;
+    }
+    life_end_setup(&self->life, ok);
+    return ok ? 0 : -1;
+}
+
+static void Ragged_dealloc(RaggedObject *self)
+{
+    ragged_free_native(self);
+    PyMem_Free(self->heights);
+    PyMem_Free(self->widths);
     Py_XDECREF(self->shapes);
     Py_TYPE(self)->tp_free((PyObject *)self);
 }
 
-static PyObject *RaggedPlan_close(RaggedPlan *self, PyObject *unused)
+static PyObject *Ragged_close(RaggedObject *self, PyObject *unused)
 {
     (void)unused;
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError,
-                        "This RaggedPlan is already initializing or executing, possibly in another thread.");
+    const int r = life_close(&self->life);
+    if (r < 0)
         return NULL;
-    }
-    ragged_release(self);
+    if (r)
+        ragged_free_native(self);
     Py_RETURN_NONE;
 }
 
-/* Validates array i of a list against shape i (the Plan rules, for a 2D array). */
-static PyArrayObject *ragged_item(PyObject *object, int h, int w, int writable)
+/* Array i of a list argument: a 2D float32 array of planned shape i. */
+static PyArrayObject *ragged_member(PyObject *o, int h, int w, int is_out)
 {
-    if (!PyArray_CheckExact(object)) {
-        PyErr_SetString(PyExc_TypeError, "Expected exact NumPy ndarrays; no implicit conversion is performed.");
+    const char *what = is_out ? "Each out array" : "Each input array";
+    PyArrayObject *a = exact_array(o, NPY_FLOAT32, what);
+    if (!a)
         return NULL;
-    }
-    PyArrayObject *a = (PyArrayObject *)object;
-    if (PyArray_TYPE(a) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(a)) {
-        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32.");
-        return NULL;
-    }
     if (PyArray_NDIM(a) != 2 || PyArray_DIM(a, 0) != h || PyArray_DIM(a, 1) != w) {
-        PyErr_SetString(PyExc_ValueError, "Each array must have the planned (height, width).");
+        PyErr_Format(PyExc_ValueError, "%s must have its planned shape (%d, %d).", what, h, w);
         return NULL;
     }
-    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
-        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, float-aligned storage.");
-        return NULL;
-    }
-    if (writable && !PyArray_ISWRITEABLE(a)) {
-        PyErr_SetString(PyExc_ValueError, "The output arrays must be writable.");
-        return NULL;
-    }
-    return a;
+    return dense_layout(a, is_out, what) ? NULL : a;
 }
 
-static PyObject *ragged_execute(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
-                                int mode)
+static PyObject *list_of(PyObject *o, const char *what)
 {
-    if (!self->batch) {
-        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is closed or uninitialized.");
+    if (!PyList_Check(o) && !PyTuple_Check(o)) {
+        PyErr_Format(PyExc_TypeError, "%s must be a list or tuple of arrays.", what);
         return NULL;
     }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This RaggedPlan is executing in another thread; use one per thread.");
+    return PySequence_Fast(o, what);
+}
+
+static PyObject *ragged_run(RaggedObject *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
+                            int mode)
+{
+    if (life_begin_call(&self->life))
         return NULL;
+    PyObject *input, *out_arg, *ins = NULL, *outs = NULL, *result = NULL;
+    if (parse_input_out(args, nargs, kwnames, &input, &out_arg) || !(ins = list_of(input, "The input")))
+        goto done;
+    if ((size_t)PySequence_Fast_GET_SIZE(ins) != self->count) {
+        PyErr_Format(PyExc_ValueError, "Expected %zu arrays, one per planned shape.", self->count);
+        goto done;
     }
-    self->busy = 1;
-    PyObject *in_seq = NULL, *out_seq = NULL, *result = NULL;
-    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
-    if (nargs != 1 || nkw > 1) {
-        PyErr_SetString(PyExc_TypeError, "Use method(arrays, *, out=None).");
-        goto fail;
-    }
-    PyObject *out_obj = Py_None;
-    if (nkw) {
-        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
-            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
-            goto fail;
-        }
-        out_obj = args[1];
-    }
-    if (!PyList_Check(args[0]) && !PyTuple_Check(args[0])) {
-        PyErr_SetString(PyExc_TypeError, "Expected a list or tuple of arrays.");
-        goto fail;
-    }
-    in_seq = PySequence_Fast(args[0], "Expected a list or tuple of arrays.");
-    if (!in_seq)
-        goto fail;
-    if ((size_t)PySequence_Fast_GET_SIZE(in_seq) != self->count) {
-        PyErr_SetString(PyExc_ValueError, "The list length must equal the number of planned shapes.");
-        goto fail;
-    }
-    if (out_obj != Py_None) {
-        if (!PyList_Check(out_obj) && !PyTuple_Check(out_obj)) {
-            PyErr_SetString(PyExc_TypeError, "out must be a list or tuple of arrays.");
-            goto fail;
-        }
-        out_seq = PySequence_Fast(out_obj, "out must be a list or tuple of arrays.");
-        if (!out_seq)
-            goto fail;
-        if ((size_t)PySequence_Fast_GET_SIZE(out_seq) != self->count) {
-            PyErr_SetString(PyExc_ValueError, "out must hold one array per planned shape.");
-            goto fail;
+    if (out_arg != Py_None) {
+        if (!(outs = list_of(out_arg, "out")))
+            goto done;
+        if ((size_t)PySequence_Fast_GET_SIZE(outs) != self->count) {
+            PyErr_Format(PyExc_ValueError, "out must hold %zu arrays, one per planned shape.", self->count);
+            goto done;
         }
     }
     for (size_t i = 0; i < self->count; i++) {
-        PyObject *shape = PyTuple_GET_ITEM(self->shapes, (Py_ssize_t)i);
-        const int h = (int)PyLong_AsLong(PyTuple_GET_ITEM(shape, 0)), w = (int)PyLong_AsLong(PyTuple_GET_ITEM(shape, 1));
-        PyArrayObject *a = ragged_item(PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i), h, w, 0);
+        PyArrayObject *a = ragged_member(PySequence_Fast_GET_ITEM(ins, (Py_ssize_t)i), self->heights[i],
+                                         self->widths[i], 0);
         if (!a)
-            goto fail;
-        if (out_seq) {
-            PyArrayObject *o = ragged_item(PySequence_Fast_GET_ITEM(out_seq, (Py_ssize_t)i), h, w, 1);
+            goto done;
+        if (outs) {
+            PyArrayObject *o = ragged_member(PySequence_Fast_GET_ITEM(outs, (Py_ssize_t)i), self->heights[i],
+                                             self->widths[i], 1);
             if (!o)
-                goto fail;
-            const uintptr_t x = (uintptr_t)PyArray_DATA(a), y = (uintptr_t)PyArray_DATA(o);
-            if ((x <= y ? y - x : x - y) < (size_t)PyArray_NBYTES(a)) {
-                PyErr_SetString(PyExc_ValueError, "Input and output arrays must not overlap.");
-                goto fail;
+                goto done;
+            if (spans_meet(a, o)) {
+                PyErr_Format(PyExc_ValueError, "out[%zu] shares memory with its input.", i);
+                goto done;
             }
         }
     }
@@ -500,15 +660,15 @@ static PyObject *ragged_execute(RaggedPlan *self, PyObject *const *args, Py_ssiz
             free(self->stage_out);
             self->stage_in = self->stage_out = NULL;
             PyErr_NoMemory();
-            goto fail;
+            goto done;
         }
     }
     for (size_t i = 0; i < self->count; i++) {
-        PyArrayObject *a = (PyArrayObject *)PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i);
+        PyArrayObject *a = (PyArrayObject *)PySequence_Fast_GET_ITEM(ins, (Py_ssize_t)i);
         memcpy(self->stage_in + bridct_batch_offset(self->batch, i), PyArray_DATA(a), (size_t)PyArray_NBYTES(a));
     }
     int status;
-    if (self->elements >= 16384) {
+    if (self->elements >= NOGIL_ELEMENTS) {
         Py_BEGIN_ALLOW_THREADS
         status = bridct_batch_apply(self->batch, mode, self->stage_in, self->stage_out);
         Py_END_ALLOW_THREADS
@@ -516,54 +676,51 @@ static PyObject *ragged_execute(RaggedPlan *self, PyObject *const *args, Py_ssiz
         status = bridct_batch_apply(self->batch, mode, self->stage_in, self->stage_out);
     }
     if (status) {
-        PyErr_Format(PyExc_RuntimeError, "BRiDCT failed with status %d.", status);
-        goto fail;
+        gpu_status_error("The batch transform", status);
+        goto done;
     }
-    result = PyList_New((Py_ssize_t)self->count);
-    if (!result)
-        goto fail;
+    PyObject *list = PyList_New((Py_ssize_t)self->count);
+    if (!list)
+        goto done;
     for (size_t i = 0; i < self->count; i++) {
         PyArrayObject *o;
-        if (out_seq) {
-            o = (PyArrayObject *)PySequence_Fast_GET_ITEM(out_seq, (Py_ssize_t)i);
+        if (outs) {
+            o = (PyArrayObject *)PySequence_Fast_GET_ITEM(outs, (Py_ssize_t)i);
             Py_INCREF(o);
         } else {
-            PyArrayObject *a = (PyArrayObject *)PySequence_Fast_GET_ITEM(in_seq, (Py_ssize_t)i);
-            o = (PyArrayObject *)PyArray_SimpleNew(2, PyArray_DIMS(a), NPY_FLOAT32);
-            if (!o)
-                goto fail;
+            const npy_intp dims[2] = {self->heights[i], self->widths[i]};
+            if (!(o = (PyArrayObject *)PyArray_SimpleNew(2, (npy_intp *)dims, NPY_FLOAT32))) {
+                Py_DECREF(list);
+                goto done;
+            }
         }
         memcpy(PyArray_DATA(o), self->stage_out + bridct_batch_offset(self->batch, i), (size_t)PyArray_NBYTES(o));
-        PyList_SET_ITEM(result, (Py_ssize_t)i, (PyObject *)o);
+        PyList_SET_ITEM(list, (Py_ssize_t)i, (PyObject *)o);
     }
-    Py_DECREF(in_seq);
-    Py_XDECREF(out_seq);
-    self->busy = 0;
+    result = list;
+done:
+    Py_XDECREF(ins);
+    Py_XDECREF(outs);
+    life_end_call(&self->life);
     return result;
-fail:
-    self->busy = 0;
-    Py_XDECREF(in_seq);
-    Py_XDECREF(out_seq);
-    Py_XDECREF(result);
-    return NULL;
 }
 
-static PyObject *RaggedPlan_forward(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static PyObject *Ragged_forward(RaggedObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
 {
-    return ragged_execute(self, args, nargs, kw, BRIDCT_FORWARD);
+    return ragged_run(self, a, n, k, BRIDCT_FORWARD);
 }
 
-static PyObject *RaggedPlan_inverse(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static PyObject *Ragged_inverse(RaggedObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
 {
-    return ragged_execute(self, args, nargs, kw, BRIDCT_INVERSE);
+    return ragged_run(self, a, n, k, BRIDCT_INVERSE);
 }
 
-static PyObject *RaggedPlan_roundtrip(RaggedPlan *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static PyObject *Ragged_roundtrip(RaggedObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
 {
-    return ragged_execute(self, args, nargs, kw, BRIDCT_GPU_FORWARD_THEN_INVERSE);
+    return ragged_run(self, a, n, k, BRIDCT_GPU_FORWARD_THEN_INVERSE);
 }
 
-static PyObject *RaggedPlan_shapes(RaggedPlan *self, void *unused)
+static PyObject *Ragged_get_shapes(RaggedObject *self, void *unused)
 {
     (void)unused;
     if (!self->shapes)
@@ -572,362 +729,284 @@ static PyObject *RaggedPlan_shapes(RaggedPlan *self, void *unused)
     return self->shapes;
 }
 
-static PyObject *RaggedPlan_closed(RaggedPlan *self, void *unused)
+static PyObject *Ragged_get_closed(RaggedObject *self, void *unused)
 {
     (void)unused;
-    return PyBool_FromLong(self->batch == NULL);
+    return PyBool_FromLong(!life_open(&self->life));
 }
 
-static PyMethodDef RaggedPlan_methods[] = {
-    {"forward", AS_METHOD(RaggedPlan_forward), METH_FASTCALL | METH_KEYWORDS,
-     "forward(arrays, *, out=None): orthonormal 2D DCT-II of every array, one GPU launch."},
-    {"inverse", AS_METHOD(RaggedPlan_inverse), METH_FASTCALL | METH_KEYWORDS,
-     "inverse(arrays, *, out=None): orthonormal 2D DCT-III of every array, one GPU launch."},
-    {"roundtrip", AS_METHOD(RaggedPlan_roundtrip), METH_FASTCALL | METH_KEYWORDS,
-     "roundtrip(arrays, *, out=None): forward then inverse with a materialized spectrum."},
-    {"close", AS_METHOD(RaggedPlan_close), METH_NOARGS, "Release the GPU batch and its buffers."},
+static PyMethodDef Ragged_methods[] = {
+    {"forward", VECTORCALL_METHOD(Ragged_forward), METH_FASTCALL | METH_KEYWORDS,
+     "forward(arrays, *, out=None) -> list: orthonormal 2D DCT-II of every array in one batch call."},
+    {"inverse", VECTORCALL_METHOD(Ragged_inverse), METH_FASTCALL | METH_KEYWORDS,
+     "inverse(arrays, *, out=None) -> list: orthonormal 2D DCT-III of every array in one batch call."},
+    {"roundtrip", VECTORCALL_METHOD(Ragged_roundtrip), METH_FASTCALL | METH_KEYWORDS,
+     "roundtrip(arrays, *, out=None) -> list: inverse(forward(a)) per array, spectrum in float32."},
+    {"close", VECTORCALL_METHOD(Ragged_close), METH_NOARGS, "close() -> None; frees the GPU batch (idempotent)."},
     {NULL, NULL, 0, NULL}};
 
-static PyGetSetDef RaggedPlan_getset[] = {
-    {"shapes", (getter)RaggedPlan_shapes, NULL, "Planned ((height, width), ...).", NULL},
-    {"closed", (getter)RaggedPlan_closed, NULL, "Whether the plan has been closed.", NULL},
+static PyGetSetDef Ragged_getset[] = {
+    {"shapes", (getter)Ragged_get_shapes, NULL, "((height, width), ...) of the plan; None before initialization.",
+     NULL},
+    {"closed", (getter)Ragged_get_closed, NULL, "True unless the plan is initialized and not yet closed.", NULL},
     {NULL, NULL, NULL, NULL, NULL}};
 
-static PyTypeObject RaggedPlanType = {
+static PyTypeObject Ragged_Type = {
     PyVarObject_HEAD_INIT(NULL, 0)
     .tp_name = "bridct_numpy.RaggedPlan",
-    .tp_basicsize = sizeof(RaggedPlan),
-    .tp_dealloc = (destructor)RaggedPlan_dealloc,
+    .tp_doc = "RaggedPlan(shapes): orthonormal float32 2D DCT of a list of arrays of mixed shapes on the GPU.",
+    .tp_basicsize = sizeof(RaggedObject),
     .tp_flags = Py_TPFLAGS_DEFAULT,
-    .tp_doc = "Orthonormal float32 2D DCT of a list of arrays of mixed shapes, one GPU launch per direction.",
-    .tp_methods = RaggedPlan_methods,
-    .tp_getset = RaggedPlan_getset,
-    .tp_init = (initproc)RaggedPlan_init,
-    .tp_new = PyType_GenericNew};
+    .tp_new = Ragged_new,
+    .tp_init = (initproc)Ragged_init,
+    .tp_dealloc = (destructor)Ragged_dealloc,
+    .tp_methods = Ragged_methods,
+    .tp_getset = Ragged_getset};
 
-/* ---- Codec8 ---------------------------------------------------------------------------- */
+/* ---- Codec8 ---------------------------------------------------------------------------------- */
 
 typedef struct {
     PyObject_HEAD
+    life life;
     bridct_codec8 *codec;
     float quant[64];
-    int initialized, busy;
-} Codec8;
+} CodecObject;
 
-static int Codec8_init(Codec8 *self, PyObject *args, PyObject *kwargs)
+static PyObject *Codec_new(PyTypeObject *type, PyObject *args, PyObject *kwargs)
 {
-    static char *keywords[] = {"quant", NULL};
-    PyObject *q_obj;
-    if (self->initialized) {
-        PyErr_SetString(PyExc_RuntimeError, "A Codec8 cannot be reinitialized.");
+    (void)args;
+    (void)kwargs;
+    CodecObject *self = (CodecObject *)type->tp_alloc(type, 0);
+    if (self)
+        self->life = (life){LIFE_NEW, "Codec8"};
+    return (PyObject *)self;
+}
+
+/* 64 finite positive float32 steps, shape (64,) or (8, 8), C order. */
+static int codec_read_steps(PyObject *o, float steps[64])
+{
+    PyArrayObject *q = exact_array(o, NPY_FLOAT32, "quant");
+    if (!q)
         return -1;
-    }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is already initializing or executing.");
-        return -1;
-    }
-    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "O:Codec8", keywords, &q_obj))
-        return -1;
-    if (!PyArray_CheckExact(q_obj)) {
-        PyErr_SetString(PyExc_TypeError, "Expected the 64 quantizer steps as an exact NumPy ndarray.");
-        return -1;
-    }
-    PyArrayObject *q = (PyArrayObject *)q_obj;
-    if (PyArray_TYPE(q) != NPY_FLOAT32 || !PyArray_ISNOTSWAPPED(q)) {
-        PyErr_SetString(PyExc_TypeError, "Expected native-endian float32 quantizer steps.");
-        return -1;
-    }
     if (PyArray_SIZE(q) != 64 || !(PyArray_NDIM(q) == 1 || (PyArray_NDIM(q) == 2 && PyArray_DIM(q, 0) == 8)) ||
-        !PyArray_IS_C_CONTIGUOUS(q) || !PyArray_ISALIGNED(q)) {
-        PyErr_SetString(PyExc_ValueError, "Expected 64 C-contiguous steps, shape (64,) or (8, 8).");
+        dense_layout(q, 0, "quant")) {
+        if (!PyErr_Occurred())
+            PyErr_SetString(PyExc_ValueError, "quant must hold 64 steps, shape (64,) or (8, 8).");
         return -1;
     }
-    const float *steps = PyArray_DATA(q);
+    memcpy(steps, PyArray_DATA(q), 64 * sizeof(float));
     for (int i = 0; i < 64; i++)
         if (!isfinite(steps[i]) || !(steps[i] > 0.0f)) {
             PyErr_SetString(PyExc_ValueError, "Every quantizer step must be finite and positive.");
             return -1;
         }
-    if (!bridct_gpu_available()) {
-        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU: the BRiDCT codec cannot be created.");
-        return -1;
-    }
-    self->busy = 1;
-    bridct_codec8 *codec = bridct_codec8_create(steps);
-    self->busy = 0;
-    if (!codec) {
-        PyErr_NoMemory();
-        return -1;
-    }
-    memcpy(self->quant, steps, sizeof(self->quant));
-    self->codec = codec;
-    self->initialized = 1;
     return 0;
 }
 
-static void Codec8_dealloc(Codec8 *self)
+static int Codec_init(CodecObject *self, PyObject *args, PyObject *kwargs)
+{
+    static char *names[] = {"quant", NULL};
+    PyObject *q = NULL;
+    float steps[64];
+    if (life_begin_setup(&self->life))
+        return -1;
+    int ok = PyArg_ParseTupleAndKeywords(args, kwargs, "O:Codec8", names, &q) && codec_read_steps(q, steps) == 0;
+    if (ok && !bridct_gpu_available()) {
+        PyErr_SetString(PyExc_RuntimeError, "No usable gfx950 GPU; this library has no CPU path.");
+        ok = 0;
+    }
+    if (ok && !(self->codec = bridct_codec8_create(steps))) {
+        PyErr_SetString(PyExc_MemoryError, "Could not allocate the GPU codec.");
+        ok = 0;
+    }
+    if (ok)
+        memcpy(self->quant, steps, sizeof(steps));
+    life_end_setup(&self->life, ok);
+    return ok ? 0 : -1;
+}
+
+static void Codec_dealloc(CodecObject *self)
 {
     bridct_codec8_destroy(self->codec);
     Py_TYPE(self)->tp_free((PyObject *)self);
 }
 
-static PyObject *Codec8_close(Codec8 *self, PyObject *unused)
+static PyObject *Codec_close(CodecObject *self, PyObject *unused)
 {
     (void)unused;
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError,
-                        "This Codec8 is already initializing or executing, possibly in another thread.");
+    const int r = life_close(&self->life);
+    if (r < 0)
         return NULL;
+    if (r) {
+        bridct_codec8_destroy(self->codec);
+        self->codec = NULL;
     }
-    bridct_codec8_destroy(self->codec);
-    self->codec = NULL;
     Py_RETURN_NONE;
 }
 
-/* A uint8 (H, W) raster with unit column stride and rows at least W bytes apart. */
-static PyArrayObject *codec_image(PyObject *object, int writable, npy_intp h, npy_intp w)
+/* A uint8 (H, W) raster, H and W positive multiples of 8, unit column stride and rows at
+ * least W bytes apart (the row stride is passed to the GPU). */
+static PyArrayObject *codec_image(PyObject *o, int is_out, const void *ctx)
 {
-    if (!PyArray_CheckExact(object)) {
-        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+    (void)ctx;
+    const char *what = is_out ? "out" : "The image";
+    PyArrayObject *a = exact_array(o, NPY_UINT8, what);
+    if (!a)
         return NULL;
-    }
-    PyArrayObject *a = (PyArrayObject *)object;
-    if (PyArray_TYPE(a) != NPY_UINT8) {
-        PyErr_SetString(PyExc_TypeError, "Expected a uint8 image.");
-        return NULL;
-    }
     if (PyArray_NDIM(a) != 2 || PyArray_DIM(a, 0) < 8 || PyArray_DIM(a, 1) < 8 || PyArray_DIM(a, 0) % 8 ||
-        PyArray_DIM(a, 1) % 8 || (h >= 0 && (PyArray_DIM(a, 0) != h || PyArray_DIM(a, 1) != w))) {
-        PyErr_SetString(PyExc_ValueError, "Expected a (height, width) image, both multiples of 8 (and matching).");
+        PyArray_DIM(a, 1) % 8) {
+        PyErr_Format(PyExc_ValueError, "%s must be (height, width) with both multiples of 8.", what);
         return NULL;
     }
     if (PyArray_STRIDE(a, 1) != 1 || PyArray_STRIDE(a, 0) < PyArray_DIM(a, 1)) {
-        PyErr_SetString(PyExc_ValueError, "Expected contiguous rows (unit column stride, forward row stride).");
+        PyErr_Format(PyExc_ValueError, "%s must have contiguous rows in increasing order.", what);
         return NULL;
     }
-    if (writable && !PyArray_ISWRITEABLE(a)) {
-        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
+    if (is_out && !PyArray_ISWRITEABLE(a)) {
+        PyErr_Format(PyExc_ValueError, "%s must be writable.", what);
         return NULL;
     }
     return a;
 }
 
-/* int16 (H/8, W/8, 8, 8) coefficients, C-contiguous. */
-static PyArrayObject *codec_coeffs(PyObject *object, int writable, npy_intp bh, npy_intp bw)
+/* int16 (H/8, W/8, 8, 8) coefficients, C order. */
+static PyArrayObject *codec_coeffs(PyObject *o, int is_out, const void *ctx)
 {
-    if (!PyArray_CheckExact(object)) {
-        PyErr_SetString(PyExc_TypeError, "Expected an exact NumPy ndarray; no implicit conversion is performed.");
+    (void)ctx;
+    const char *what = is_out ? "out" : "The coefficients";
+    PyArrayObject *a = exact_array(o, NPY_INT16, what);
+    if (!a)
         return NULL;
-    }
-    PyArrayObject *a = (PyArrayObject *)object;
-    if (PyArray_TYPE(a) != NPY_INT16 || !PyArray_ISNOTSWAPPED(a)) {
-        PyErr_SetString(PyExc_TypeError, "Expected native-endian int16 coefficients.");
-        return NULL;
-    }
     if (PyArray_NDIM(a) != 4 || PyArray_DIM(a, 0) < 1 || PyArray_DIM(a, 1) < 1 || PyArray_DIM(a, 2) != 8 ||
-        PyArray_DIM(a, 3) != 8 || (bh >= 0 && (PyArray_DIM(a, 0) != bh || PyArray_DIM(a, 1) != bw))) {
-        PyErr_SetString(PyExc_ValueError, "Expected (height/8, width/8, 8, 8) coefficients (and matching).");
+        PyArray_DIM(a, 3) != 8) {
+        PyErr_Format(PyExc_ValueError, "%s must have shape (height/8, width/8, 8, 8).", what);
         return NULL;
     }
-    if (!PyArray_IS_C_CONTIGUOUS(a) || !PyArray_ISALIGNED(a)) {
-        PyErr_SetString(PyExc_ValueError, "Expected C-contiguous, aligned coefficients.");
-        return NULL;
-    }
-    if (writable && !PyArray_ISWRITEABLE(a)) {
-        PyErr_SetString(PyExc_ValueError, "The output array must be writable.");
-        return NULL;
-    }
-    return a;
+    return dense_layout(a, is_out, what) ? NULL : a;
 }
 
-/* First and one-past-last byte of an array's storage (non-negative strides). */
-static void extent(PyArrayObject *a, uintptr_t *lo, uintptr_t *hi)
+static PyObject *codec_run(CodecObject *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
+                           int decode)
 {
-    uintptr_t end = (uintptr_t)PyArray_DATA(a);
-    for (int d = 0; d < PyArray_NDIM(a); d++)
-        end += (uintptr_t)(PyArray_DIM(a, d) - 1) * (uintptr_t)PyArray_STRIDE(a, d);
-    *lo = (uintptr_t)PyArray_DATA(a);
-    *hi = end + (uintptr_t)PyArray_ITEMSIZE(a);
-}
-
-static PyObject *codec_execute(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kwnames,
-                               int decode)
-{
-    if (!self->codec) {
-        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is closed or uninitialized.");
+    if (life_begin_call(&self->life))
         return NULL;
-    }
-    if (self->busy) {
-        PyErr_SetString(PyExc_RuntimeError, "This Codec8 is executing in another thread; use one per thread.");
-        return NULL;
-    }
-    self->busy = 1;
-    PyArrayObject *out = NULL;
-    const Py_ssize_t nkw = kwnames ? PyTuple_GET_SIZE(kwnames) : 0;
-    if (nargs != 1 || nkw > 1) {
-        PyErr_SetString(PyExc_TypeError, "Use method(input, *, out=None).");
-        goto fail;
-    }
-    PyObject *out_obj = Py_None;
-    if (nkw) {
-        if (PyUnicode_CompareWithASCIIString(PyTuple_GET_ITEM(kwnames, 0), "out") != 0) {
-            PyErr_SetString(PyExc_TypeError, "The only accepted keyword is out.");
-            goto fail;
-        }
-        out_obj = args[1];
-    }
-    PyArrayObject *in = decode ? codec_coeffs(args[0], 0, -1, -1) : codec_image(args[0], 0, -1, -1);
-    if (!in)
-        goto fail;
+    PyObject *input, *out_arg, *result = NULL;
+    PyArrayObject *in, *out = NULL;
+    if (parse_input_out(args, nargs, kwnames, &input, &out_arg))
+        goto done;
+    if (!(in = decode ? codec_coeffs(input, 0, NULL) : codec_image(input, 0, NULL)))
+        goto done;
     const npy_intp h = decode ? 8 * PyArray_DIM(in, 0) : PyArray_DIM(in, 0);
     const npy_intp w = decode ? 8 * PyArray_DIM(in, 1) : PyArray_DIM(in, 1);
-    if (out_obj == Py_None) {
-        npy_intp img_dims[2] = {h, w}, co_dims[4] = {h / 8, w / 8, 8, 8};
-        out = (PyArrayObject *)(decode ? PyArray_SimpleNew(2, img_dims, NPY_UINT8)
-                                       : PyArray_SimpleNew(4, co_dims, NPY_INT16));
-        if (!out)
-            goto fail;
-    } else {
-        PyArrayObject *given = decode ? codec_image(out_obj, 1, h, w) : codec_coeffs(out_obj, 1, h / 8, w / 8);
-        if (!given)
-            goto fail;
-        uintptr_t a0, a1, b0, b1;
-        extent(in, &a0, &a1);
-        extent(given, &b0, &b1);
-        if (a0 < b1 && b0 < a1) {
-            PyErr_SetString(PyExc_ValueError, "Input and output must not overlap.");
-            goto fail;
-        }
-        out = given;
-        Py_INCREF(out);
-    }
-    PyArrayObject *image = decode ? out : in;
-    const size_t stride = (size_t)PyArray_STRIDE(image, 0);
+    const npy_intp img_dims[2] = {h, w}, coeff_dims[4] = {h / 8, w / 8, 8, 8};
+    out = decode ? output_for(out_arg, in, 2, img_dims, NPY_UINT8, codec_image, NULL)
+                 : output_for(out_arg, in, 4, coeff_dims, NPY_INT16, codec_coeffs, NULL);
+    if (!out)
+        goto done;
+    const size_t stride = (size_t)PyArray_STRIDE(decode ? out : in, 0);
     int status;
     Py_BEGIN_ALLOW_THREADS
-    if (decode)
-        status = bridct_codec8_decode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
-                                      PyArray_DATA(out));
-    else
-        status = bridct_codec8_encode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
-                                      PyArray_DATA(out));
+    status = decode ? bridct_codec8_decode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
+                                           PyArray_DATA(out))
+                    : bridct_codec8_encode(self->codec, PyArray_DATA(in), (size_t)h, (size_t)w, stride,
+                                           PyArray_DATA(out));
     Py_END_ALLOW_THREADS
     if (status) {
-        PyErr_Format(PyExc_RuntimeError, "BRiDCT codec failed with status %d.", status);
-        goto fail;
+        gpu_status_error(decode ? "Decoding" : "Encoding", status);
+        goto done;
     }
-    self->busy = 0;
-    return (PyObject *)out;
-fail:
-    self->busy = 0;
+    result = (PyObject *)out;
+    out = NULL;
+done:
     Py_XDECREF(out);
-    return NULL;
+    life_end_call(&self->life);
+    return result;
 }
 
-static PyObject *Codec8_encode(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static PyObject *Codec_encode(CodecObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
 {
-    return codec_execute(self, args, nargs, kw, 0);
+    return codec_run(self, a, n, k, 0);
 }
 
-static PyObject *Codec8_decode(Codec8 *self, PyObject *const *args, Py_ssize_t nargs, PyObject *kw)
+static PyObject *Codec_decode(CodecObject *self, PyObject *const *a, Py_ssize_t n, PyObject *k)
 {
-    return codec_execute(self, args, nargs, kw, 1);
+    return codec_run(self, a, n, k, 1);
 }
 
-static PyObject *Codec8_quant(Codec8 *self, void *unused)
+static PyObject *Codec_get_quant(CodecObject *self, void *unused)
 {
     (void)unused;
-    if (!self->initialized)
+    if (self->life.state == LIFE_NEW || self->life.state == LIFE_SETUP)
         Py_RETURN_NONE;
-    npy_intp dims[2] = {8, 8};
-    PyObject *q = PyArray_SimpleNew(2, dims, NPY_FLOAT32);
+    const npy_intp dims[2] = {8, 8};
+    PyObject *q = PyArray_SimpleNew(2, (npy_intp *)dims, NPY_FLOAT32);
     if (q)
         memcpy(PyArray_DATA((PyArrayObject *)q), self->quant, sizeof(self->quant));
     return q;
 }
 
-static PyObject *Codec8_closed(Codec8 *self, void *unused)
+static PyObject *Codec_get_closed(CodecObject *self, void *unused)
 {
     (void)unused;
-    return PyBool_FromLong(self->codec == NULL);
+    return PyBool_FromLong(!life_open(&self->life));
 }
 
-static PyMethodDef Codec8_methods[] = {
-    {"encode", AS_METHOD(Codec8_encode), METH_FASTCALL | METH_KEYWORDS,
-     "encode(image, *, out=None): uint8 (H, W) -> int16 (H/8, W/8, 8, 8) quantized 8x8 DCT coefficients."},
-    {"decode", AS_METHOD(Codec8_decode), METH_FASTCALL | METH_KEYWORDS,
-     "decode(coeffs, *, out=None): int16 (H/8, W/8, 8, 8) -> uint8 (H, W) image."},
-    {"close", AS_METHOD(Codec8_close), METH_NOARGS, "Release the GPU codec and its buffers."},
+static PyMethodDef Codec_methods[] = {
+    {"encode", VECTORCALL_METHOD(Codec_encode), METH_FASTCALL | METH_KEYWORDS,
+     "encode(image, *, out=None) -> int16 (H/8, W/8, 8, 8) quantized 8x8 DCT coefficients of a uint8 image."},
+    {"decode", VECTORCALL_METHOD(Codec_decode), METH_FASTCALL | METH_KEYWORDS,
+     "decode(coeffs, *, out=None) -> uint8 (H, W) image from quantized coefficients."},
+    {"close", VECTORCALL_METHOD(Codec_close), METH_NOARGS, "close() -> None; frees the GPU codec (idempotent)."},
     {NULL, NULL, 0, NULL}};
 
-static PyGetSetDef Codec8_getset[] = {
-    {"quant", (getter)Codec8_quant, NULL, "The quantizer steps, (8, 8) float32.", NULL},
-    {"closed", (getter)Codec8_closed, NULL, "Whether the codec has been closed.", NULL},
+static PyGetSetDef Codec_getset[] = {
+    {"quant", (getter)Codec_get_quant, NULL, "The 64 quantizer steps as (8, 8) float32; None before initialization.",
+     NULL},
+    {"closed", (getter)Codec_get_closed, NULL, "True unless the codec is initialized and not yet closed.", NULL},
     {NULL, NULL, NULL, NULL, NULL}};
 
-static PyTypeObject Codec8Type = {
+static PyTypeObject Codec_Type = {
     PyVarObject_HEAD_INIT(NULL, 0)
     .tp_name = "bridct_numpy.Codec8",
-    .tp_basicsize = sizeof(Codec8),
-    .tp_dealloc = (destructor)Codec8_dealloc,
+    .tp_doc = "Codec8(quant): 8x8 block codec step on the GPU (level shift, 8x8 DCT, quantization, and back).",
+    .tp_basicsize = sizeof(CodecObject),
     .tp_flags = Py_TPFLAGS_DEFAULT,
-    .tp_doc = "8x8 block codec step on the GPU: level shift, 8x8 DCT and quantization, and back.",
-    .tp_methods = Codec8_methods,
-    .tp_getset = Codec8_getset,
-    .tp_init = (initproc)Codec8_init,
-    .tp_new = PyType_GenericNew};
+    .tp_new = Codec_new,
+    .tp_init = (initproc)Codec_init,
+    .tp_dealloc = (destructor)Codec_dealloc,
+    .tp_methods = Codec_methods,
+    .tp_getset = Codec_getset};
 
-static int module_exec(PyObject *m)
+/* ---- module ---------------------------------------------------------------------------------- */
+
+static int bridct_numpy_exec(PyObject *module)
 {
+    /* The types are static and each object's state is guarded by one GIL: another
+     * interpreter would share the types without sharing that guard. */
     if (PyInterpreterState_Get() != PyInterpreterState_Main()) {
-        PyErr_SetString(PyExc_ImportError, "bridct_numpy supports the main interpreter only.");
+        PyErr_SetString(PyExc_ImportError,
+                        "bridct_numpy loads in the main interpreter only; subinterpreters are not supported.");
         return -1;
     }
     if (_import_array() < 0)
         return -1;
-    if (PyType_Ready(&PlanType) < 0)
-        return -1;
-    Py_INCREF(&PlanType);
-    if (PyModule_AddObject(m, "Plan", (PyObject *)&PlanType) < 0) {
-        Py_DECREF(&PlanType);
-        return -1;
-    }
-    if (PyType_Ready(&RaggedPlanType) < 0)
-        return -1;
-    Py_INCREF(&RaggedPlanType);
-    if (PyModule_AddObject(m, "RaggedPlan", (PyObject *)&RaggedPlanType) < 0) {
-        Py_DECREF(&RaggedPlanType);
-        return -1;
-    }
-    if (PyType_Ready(&Codec8Type) < 0)
-        return -1;
-    Py_INCREF(&Codec8Type);
-    if (PyModule_AddObject(m, "Codec8", (PyObject *)&Codec8Type) < 0) {
-        Py_DECREF(&Codec8Type);
-        return -1;
-    }
-    if (PyModule_AddStringConstant(m, "core_version", BRIDCT_VERSION) < 0)
-        return -1;
-    return 0;
+    PyTypeObject *types[] = {&Plan_Type, &Ragged_Type, &Codec_Type};
+    for (size_t i = 0; i < sizeof(types) / sizeof(types[0]); i++)
+        if (PyModule_AddType(module, types[i]) < 0)
+            return -1;
+    return PyModule_AddStringConstant(module, "core_version", BRIDCT_VERSION);
 }
 
-#if defined(__GNUC__) && !defined(__clang__)
-#pragma GCC diagnostic push
-#pragma GCC diagnostic ignored "-Wpedantic"
-#endif
-static PyModuleDef_Slot module_slots[] = {{Py_mod_exec, (void *)module_exec}, {0, NULL}};
-#if defined(__GNUC__) && !defined(__clang__)
-#pragma GCC diagnostic pop
-#endif
+static PyModuleDef_Slot bridct_numpy_slots[] = {{Py_mod_exec, (void *)bridct_numpy_exec}, {0, NULL}};
 
-static PyModuleDef module_def = {
+static struct PyModuleDef bridct_numpy_module = {
     PyModuleDef_HEAD_INIT,
     .m_name = "bridct_numpy",
-    .m_doc = "NumPy interface to the MI355X BRiDCT 2D DCT plan library.",
+    .m_doc = "NumPy interface to the MI355X BRiDCT library: Plan, RaggedPlan, Codec8, core_version.",
     .m_size = 0,
-    .m_slots = module_slots};
+    .m_slots = bridct_numpy_slots};
 
 PyMODINIT_FUNC PyInit_bridct_numpy(void)
 {
-    return PyModuleDef_Init(&module_def);
+    return PyModuleDef_Init(&bridct_numpy_module);
 }

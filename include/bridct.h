@@ -65,7 +65,9 @@ BRIDCT_API int bridct_plan_apply(bridct_plan *plan, enum bridct_mode mode,
 BRIDCT_API size_t bridct_workspace_floats(int size);
 
 /* Replaces bridct_apply (include/bridct.h:48-50, src/bridct.c:30-75): square 8..256 with
- * caller workspace, error precedence -1, -2, -3, -4. */
+ * caller workspace, error precedence -1, -2, -3, -4.  The GPU state of each size (tables,
+ * stream, staging) is made by a thread's first call of that size and kept until the
+ * thread exits; BRIDCT_DEVICE_ERROR when it cannot be made. */
 BRIDCT_API int bridct_apply(int size, enum bridct_mode mode,
                             const float *input, float *output,
                             float *workspace, size_t workspace_count);

@@ -164,6 +164,15 @@ BRIDCT_API void *bridct_codec8_stream(const bridct_codec8 *codec);
 BRIDCT_API int bridct_batch_set_strategy(bridct_batch *batch, int strategy);
 BRIDCT_API int bridct_batch_get_strategy(const bridct_batch *batch);
 
+/* Failure injection, for tests of the allocation-failure paths (plan, batch and codec
+ * creation, and the buffers some calls create on first use).  after >= 0: the library's
+ * next `after` GPU-runtime acquisitions (device and pinned host buffers, streams, events)
+ * succeed and the one after fails as if the runtime had refused it; then injection is off
+ * again.  after < 0: off.  Returns the number of such resources the library holds at the
+ * call, so a test can check that a failed creation released what it had acquired.
+ * Process-wide; not for concurrent use with other library calls. */
+BRIDCT_API long bridct_debug_alloc_faults(long after);
+
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);
 BRIDCT_API int bridct_gpu_free(void *d_ptr);

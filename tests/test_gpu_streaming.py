@@ -4,7 +4,8 @@ Calls whose input plus output exceed 256 MiB (the Infinity Cache) run non-tempor
 instantiations of the pass kernels and of the tiny batch kernel: the forms every
 large-batch throughput figure comes from.  Each test runs one such call at the size the
 benchmark uses (or just past the threshold), compares a sample of arrays per shape with
-the FP64 oracle (2x the reference CPU's error level, as in the 64-shape parity gate), and
+the FP64 oracle (at most 2x the reference CPU library's max-relative error on the same
+array, the rule of the 64-shape parity gate; a fixed 4.5e-7 for the 8x8 blocks), and
 compares the same arrays bitwise with a call small enough to take the cached forms (the
 arithmetic is the same; only the load/store hints differ).  Contract: reference
 src/plan.c:126-163 applied per array.
@@ -19,9 +20,18 @@ from bridct_amd import capi
 pytestmark = pytest.mark.gpu
 
 STREAM_MIN_BYTES = 256 << 20
-# 2x the reference CPU's worst max-relative error against FP64 over the 64 shapes
-# (forward 2.23e-7, inverse 2.14e-7, measured on the reference built from source).
+# 2x the reference CPU's worst max-relative error against FP64 over the 64 shapes on
+# uniform inputs (forward 2.23e-7, inverse 2.14e-7): the fixed bar of the 8x8 blocks.
 FWD_INV_TOL = 4.5e-7
+
+
+def gate(y, x, mode):
+    """max-rel error of y against FP64 at most twice the reference CPU's on x (parity rule)."""
+    exact = S.oracle_2d(x, mode)
+    e_gpu = S.max_rel(y, exact)
+    e_ref = S.max_rel(S.ref_apply(np.ascontiguousarray(x, np.float32), mode), exact)
+    assert e_gpu <= 2.0 * e_ref, (x.shape, mode, e_gpu, e_ref)
+    return e_gpu
 
 
 def device_call(plan, mode, x, batch, in_place=False):
@@ -55,9 +65,7 @@ def test_1024_streaming_batch_against_oracle(batch, in_place):
             small = device_call(plan, mode, np.ascontiguousarray(x[sample]), len(sample))
             assert np.array_equal(y[sample], small), 'streaming and cached forms differ'
             for i in sample:
-                ref = S.oracle_2d(x[i], mode)
-                err = S.max_rel(y[i], ref)
-                assert err < (FWD_INV_TOL if mode < 2 else 1.5e-6), (mode, i, err)
+                gate(y[i], x[i], mode)
             assert np.isfinite(y).all()
     finally:
         plan.close()
@@ -97,12 +105,8 @@ def test_ragged_10000_grouped_streaming_against_oracle():
             o = b.offsets[i]
             a = x[o:o + h * w].reshape(h, w)
             got = y[o:o + h * w].reshape(h, w)
-            ref = S.oracle_2d(a, 0)
-            err = S.max_rel(got, ref)
-            worst = max(worst, err)
-            assert err < FWD_INV_TOL, ((h, w), err)
-            back = S.oracle_2d(got, 1)
-            assert S.max_rel(z[o:o + h * w].reshape(h, w), back) < FWD_INV_TOL, (h, w)
+            worst = max(worst, gate(got, a, 0))
+            gate(z[o:o + h * w].reshape(h, w), got, 1)
             p = plans.setdefault((h, w), capi.Plan(h, w))
             one = np.empty_like(a)
             assert p.apply_batch(capi.FORWARD, np.ascontiguousarray(a), one) == 0
