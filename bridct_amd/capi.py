@@ -32,6 +32,8 @@ SIGNATURES = {
     'bridct_gpu_available': (ctypes.c_int, []),
     'bridct_plan_apply_device': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_size_t]),
+    'bridct_plan_apply_device_static': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_apply_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_size_t]),
     'bridct_plan_apply_device_pass': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,

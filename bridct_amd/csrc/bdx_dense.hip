@@ -21,6 +21,8 @@
 // column l % 32.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <type_traits>
 
 #include "bridct_device.h"
@@ -246,6 +248,41 @@ int gemm_any(int prec, bool bt, const float *A, const float *B, float *C, int M,
 
 }  // namespace
 }  // namespace bdx
+
+// ---- static intermediate for calls that allocate nothing ----------------------------------
+// One file-scope array per GPU (each device holds its own copy until the process ends).
+// Host code never passes `scratch` itself to a kernel -- in host code the name is only a
+// stand-in -- but the address hipGetSymbolAddress returns for it on that device.
+static __device__ __align__(256) float scratch[BDX_STATIC_WORK_FLOATS];
+namespace {
+constexpr int kMaxDevices = 64;
+float *scratch_on[kMaxDevices];   // per device: the address of its copy of `scratch`
+std::mutex scratch_lock;
+}  // namespace
+
+extern "C" int bdx_dense_static_work(int device, float **ptr, size_t *floats)
+{
+    if (device < 0 || device >= kMaxDevices || !ptr || !floats)
+        return -1;
+    std::lock_guard<std::mutex> hold(scratch_lock);
+    if (!scratch_on[device]) {
+        int current = -1;
+        if (hipGetDevice(&current) != hipSuccess)
+            return -1;
+        if (current != device && hipSetDevice(device) != hipSuccess)
+            return -1;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device && hipSetDevice(current) != hipSuccess)
+            return -1;
+        if (e != hipSuccess || !p)
+            return -1;
+        scratch_on[device] = static_cast<float *>(p);
+    }
+    *ptr = scratch_on[device];
+    *floats = BDX_STATIC_WORK_FLOATS;
+    return 0;
+}
 
 extern "C" int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, int w, size_t batch,
                                    int mode, int prec, const float *d_qh, const float *d_qht, const float *d_qw,

@@ -103,6 +103,11 @@ int bdx_device_ready(void)
     return strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 0 : -2;
 }
 
+int bdx_current_device(int *device)
+{
+    return hipGetDevice(device) == hipSuccess ? 0 : -1;
+}
+
 int bdx_malloc(void **ptr, size_t bytes)
 {
     return acquired(!injected_fault() && hipMalloc(ptr, bytes) == hipSuccess);

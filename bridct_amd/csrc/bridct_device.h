@@ -127,6 +127,12 @@ int bdx_graph_destroy(void *graph_exec);
 #define BDX_ROUNDTRIP 2
 
 int bdx_device_ready(void);
+int bdx_current_device(int *device);
+/* The dense route's static intermediate on `device`: a file-scope device array of
+ * *floats floats (BDX_STATIC_WORK_FLOATS), shared by every call on that GPU that uses it;
+ * its address is resolved once per device.  0, or -1 when it cannot be resolved. */
+#define BDX_STATIC_WORK_FLOATS ((size_t)8 << 20)
+int bdx_dense_static_work(int device, float **ptr, size_t *floats);
 /* Failure injection and resource count (bridct_debug_alloc_faults in bridct_gpu.h). */
 long bdx_debug_alloc_faults(long after);
 int bdx_malloc(void **ptr, size_t bytes);

@@ -58,6 +58,7 @@ struct bridct_plan {
     float dense_scale;    /* 1 / (s_h s_w) */
     float *d_work;        /* dense route intermediate */
     size_t work_floats;
+    int device;           /* HIP device current at creation */
 };
 
 /* src/plan.c:33-37 */
@@ -185,6 +186,8 @@ bridct_plan *bridct_plan_create(int height, int width)
     plan->chunk = STAGE_FLOATS / plan->elements;
     if (plan->chunk < 1)
         plan->chunk = 1;
+    if (bdx_current_device(&plan->device))
+        goto fail;
     float *host = malloc(BDX_TABLE_FLOATS * sizeof(float));
     if (!host)
         goto fail;
@@ -304,11 +307,12 @@ int bridct_plan_get_pack(const bridct_plan *plan)
     return plan ? plan->pack : BRIDCT_INVALID_ARGUMENT;
 }
 
-/* Dense route: FORWARD / INVERSE through the plan's workspace (grown on demand). */
-static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+/* Dense route: FORWARD / INVERSE through `work` (count * elements floats), or through the
+ * plan's own workspace (grown on demand) when work is NULL. */
+static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count, float *work)
 {
     const size_t need = count * plan->elements;
-    if (need > plan->work_floats) {
+    if (!work && need > plan->work_floats) {
         if (plan->d_work && (bdx_stream_sync(plan->stream) || bdx_free(plan->d_work)))
             return -1;
         plan->d_work = NULL;
@@ -317,6 +321,8 @@ static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, s
             return -1;
         plan->work_floats = need;
     }
+    if (!work)
+        work = plan->d_work;
     const size_t hh = (size_t)plan->height * plan->height;
     const int prec = plan->route == BRIDCT_ROUTE_MFMA_F16 ? BDX_DENSE_F16 : BDX_DENSE_BF16;
     /* batches beyond one launch's grid limits go in slices */
@@ -325,7 +331,7 @@ static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, s
     for (size_t done = 0; done < count;) {
         const size_t n = count - done < max_batch ? count - done : max_batch;
         const size_t off = done * plan->elements;
-        if (bdx_dense_transform(d_in + off, d_out + off, plan->d_work, plan->height, plan->width, n, mode, prec,
+        if (bdx_dense_transform(d_in + off, d_out + off, work, plan->height, plan->width, n, mode, prec,
                                 plan->d_dense, plan->d_dense + hh, plan->d_dense + 2 * hh, plan->dense_scale,
                                 plan->stream))
             return -1;
@@ -334,23 +340,25 @@ static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, s
     return 0;
 }
 
-static int transform_mode(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+/* `work`: the dense route's intermediate (NULL: the plan's own). */
+static int transform_mode(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count, float *work)
 {
     if (plan->route == BRIDCT_ROUTE_BUTTERFLY)
         return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, mode, plan->d_tables,
                              plan->stream);
     if (mode == BDX_ROUNDTRIP)
-        return dense(plan, BDX_FORWARD, d_in, d_out, count) || dense(plan, BDX_INVERSE, d_out, d_out, count);
-    return dense(plan, mode, d_in, d_out, count);
+        return dense(plan, BDX_FORWARD, d_in, d_out, count, work) ||
+               dense(plan, BDX_INVERSE, d_out, d_out, count, work);
+    return dense(plan, mode, d_in, d_out, count, work);
 }
 
-static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+static int transform_with(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count, float *work)
 {
     if (plan->route != BRIDCT_ROUTE_BUTTERFLY) {
         if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
-            return transform_mode(plan, BDX_FORWARD, d_in, d_out, count) ||
-                   transform_mode(plan, BDX_INVERSE, d_out, d_out, count);
-        return transform_mode(plan, mode, d_in, d_out, count);
+            return transform_mode(plan, BDX_FORWARD, d_in, d_out, count, work) ||
+                   transform_mode(plan, BDX_INVERSE, d_out, d_out, count, work);
+        return transform_mode(plan, mode, d_in, d_out, count, work);
     }
     if (mode == BRIDCT_GPU_FORWARD_THEN_INVERSE)
         return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, BDX_FORWARD,
@@ -359,6 +367,11 @@ static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_ou
                              plan->d_tables, plan->stream);
     return bdx_transform(d_in, d_out, plan->height, plan->width, count, plan->pack, mode, plan->d_tables,
                          plan->stream);
+}
+
+static int transform(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count)
+{
+    return transform_with(plan, mode, d_in, d_out, count, NULL);
 }
 
 static int pipeline_ready(bridct_plan *plan)
@@ -479,6 +492,32 @@ int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode, const flo
         distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
         return BRIDCT_OVERLAPPING_BUFFERS;
     if (transform(plan, (int)mode, d_input, d_output, batch))
+        return BRIDCT_DEVICE_ERROR;
+    return BRIDCT_OK;
+}
+
+/* The same call with nothing allocated: a dense-route plan takes its intermediate from the
+ * library's static per-GPU scratch array instead of growing its own buffer. */
+int bridct_plan_apply_device_static(bridct_plan *plan, enum bridct_mode mode, const float *d_input,
+                                    float *d_output, size_t batch)
+{
+    if (!plan || !d_input || !d_output || (unsigned)mode > BRIDCT_ROUNDTRIP ||
+        batch > SIZE_MAX / sizeof(float) / plan->elements)
+        return BRIDCT_INVALID_ARGUMENT;
+    if (((uintptr_t)d_input | (uintptr_t)d_output) % 16)
+        return BRIDCT_BAD_ALIGNMENT;
+    if (d_input != d_output &&
+        distance_below(d_input, d_output, batch * plan->elements * sizeof(float)))
+        return BRIDCT_OVERLAPPING_BUFFERS;
+    float *work = NULL;
+    if (plan->route != BRIDCT_ROUTE_BUTTERFLY) {
+        size_t floats = 0;
+        if (bdx_dense_static_work(plan->device, &work, &floats))
+            return BRIDCT_DEVICE_ERROR;
+        if (batch * plan->elements > floats)
+            return BRIDCT_WORKSPACE_TOO_SMALL;
+    }
+    if (transform_with(plan, (int)mode, d_input, d_output, batch, work))
         return BRIDCT_DEVICE_ERROR;
     return BRIDCT_OK;
 }

@@ -25,6 +25,19 @@ BRIDCT_API int bridct_gpu_available(void);
 BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode,
                                         const float *d_input, float *d_output, size_t batch);
 
+/* bridct_plan_apply_device with nothing allocated.  No counterpart in the reference: for
+ * callers that cannot allocate device memory during the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The butterfly route
+ * needs no memory beyond the plan's; a dense-route plan (bridct_plan_set_route) takes its
+ * intermediate from one static array per GPU of 8,388,608 floats (32 MiB, held by each GPU
+ * the library runs on until the process ends) instead of allocating its own buffer, so
+ * batch * height * width may not exceed 8,388,608 there (else BRIDCT_WORKSPACE_TOO_SMALL,
+ * nothing run).  Every such call on one GPU shares that array: dense-route calls that can
+ * overlap on one GPU (other streams, other threads) race and must be serialized by the
+ * caller.  Results equal bridct_plan_apply_device's bitwise. */
+BRIDCT_API int bridct_plan_apply_device_static(bridct_plan *plan, enum bridct_mode mode,
+                                               const float *d_input, float *d_output, size_t batch);
+
 /* Routes of a plan.  BUTTERFLY (default): the FP32 Shao-Johnson butterflies.
  * MFMA_F16 / MFMA_BF16: the dense route Y = C_h X C_w^T on the matrix cores, both
  * operands split into hi + lo half-precision parts (FP16 or BF16), three products, FP32
