@@ -32,6 +32,7 @@ from . import capi
 LENGTHS = (8, 16, 32, 64, 128, 256, 512, 1024)
 SMALL_LDS_FLOATS = 36864      # whole-array route: h (w + 1) floats of LDS at most
 PACKS = (0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
+GATE_ARRAYS = 4               # arrays whose error gates a route
 ROUTE_NAMES = {capi.ROUTE_BUTTERFLY: 'butterfly', capi.ROUTE_MFMA_F16: 'mfma_f16',
                capi.ROUTE_MFMA_BF16: 'mfma_bf16'}
 
@@ -87,12 +88,17 @@ def candidates(h, w, batch):
 
 
 def tune_shape(ev, h, w, pixels):
-    batch = max(2, min(65535, pixels // (h * w)))
-    x = np.random.default_rng(h * 1024 + w).uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    batch = max(GATE_ARRAYS, min(65535, pixels // (h * w)))
+    rng = np.random.default_rng(h * 1024 + w)
+    x = rng.uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    # the arrays the error gate reads carry the ragged workload's per-array scales,
+    # 10^U[-3, 3], so a route must hold its accuracy across that range
+    for i in range(GATE_ARRAYS):
+        x[i] *= np.float32(10 ** rng.uniform(-3, 3))
     d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
     d_in.upload(x)
     qh, qw = dct_fp64(h), dct_fp64(w)
-    ref = [(qh @ x[i].astype(np.float64) @ qw.T, qh.T @ x[i].astype(np.float64) @ qw) for i in range(2)]
+    ref = [(qh @ x[i].astype(np.float64) @ qw.T, qh.T @ x[i].astype(np.float64) @ qw) for i in range(GATE_ARRAYS)]
     plan = capi.Plan(h, w)
     results = []
     try:
@@ -104,9 +110,9 @@ def tune_shape(ev, h, w, pixels):
             for mode in (capi.FORWARD, capi.INVERSE):
                 ms[mode] = ev.time_ms(plan.stream, lambda: plan.apply_device(mode, d_in.ptr, d_out.ptr, batch), 3)
                 assert plan.synchronize() == 0
-                y = np.empty_like(x[:2])
+                y = np.empty_like(x[:GATE_ARRAYS])
                 d_out.download(y)
-                for i in range(2):
+                for i in range(GATE_ARRAYS):
                     r = ref[i][mode]
                     err = max(err, float(np.abs(y[i] - r).max() / np.abs(r).max()))
             gpx = 2 * x.size / ((ms[0] + ms[1]) * 1e-3) / 1e9
@@ -136,7 +142,8 @@ def main(argv=None):
     lines = ['# BRiDCT schedule table for gfx950 (MI355X): <height> <width> <route> <pack>',
              '# written by: python3 -m bridct_amd.autotune --pixels %d  (%s UTC)' % (
                  args.pixels, datetime.datetime.utcnow().strftime('%Y-%m-%d %H:%M')),
-             '# gate: max relative error vs FP64 <= 2 x the butterfly route (pack 0) on the same arrays;',
+             '# gate: max relative error vs FP64 <= 2 x the butterfly route (pack 0) on the same %d arrays,' % GATE_ARRAYS,
+             '# scaled by 10^U[-3, 3] each;',
              '# then the highest forward+inverse Gpixel/s.  Comment: every candidate as route/pack:Gpx/s:error.']
     t0 = time.time()
     for h, w in shapes:

@@ -329,8 +329,8 @@ static int dense_setup(bridct_batch *b)
         floats += 2 * (size_t)g->h * g->h + (size_t)g->w * g->w;
         g->dscale = 1.0f / (float)(bridct_dense_scale(g->h) * bridct_dense_scale(g->w));
         const size_t n = g->count < dense_slice(g->h) ? g->count : dense_slice(g->h);
-        if (n * g->h * g->w > work)
-            work = n * g->h * g->w;
+        if (BDX_DENSE_WORK_FLOATS(n, g->h, g->w) > work)
+            work = BDX_DENSE_WORK_FLOATS(n, g->h, g->w);
     }
     float *host = malloc(floats * sizeof(float));
     if (!host)

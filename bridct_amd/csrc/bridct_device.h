@@ -129,9 +129,11 @@ int bdx_graph_destroy(void *graph_exec);
 int bdx_device_ready(void);
 int bdx_current_device(int *device);
 /* The dense route's static intermediate on `device`: a file-scope device array of
- * *floats floats (BDX_STATIC_WORK_FLOATS), shared by every call on that GPU that uses it;
+ * *floats floats (BDX_STATIC_WORK_FLOATS: BDX_STATIC_ARRAY_FLOATS of array data plus room
+ * for the per-array prescale exponents), shared by every call on that GPU that uses it;
  * its address is resolved once per device.  0, or -1 when it cannot be resolved. */
-#define BDX_STATIC_WORK_FLOATS ((size_t)8 << 20)
+#define BDX_STATIC_ARRAY_FLOATS ((size_t)8 << 20)
+#define BDX_STATIC_WORK_FLOATS (BDX_STATIC_ARRAY_FLOATS + 65536)
 int bdx_dense_static_work(int device, float **ptr, size_t *floats);
 /* Failure injection and resource count (bridct_debug_alloc_faults in bridct_gpu.h). */
 long bdx_debug_alloc_faults(long after);
@@ -170,14 +172,16 @@ int bdx_transform_gather(const float *d_in, float *d_out, int h, int w, size_t c
 /* Y = scale Q_h' X Q_w'^T (FORWARD) or X = scale Q_h'^T Y Q_w' (INVERSE) for `batch`
  * contiguous h x w arrays, h, w >= 64, batch <= 65535 and batch h / 64 <= 65535.
  * d_qh, d_qht, d_qw: the scaled DCT-II matrices Q_h' (h x h), its transpose, Q_w' (w x w),
- * row-major float32; d_work: batch h w floats for the intermediate; in place allowed. */
+ * row-major float32; d_work: BDX_DENSE_WORK_FLOATS(batch, h, w) floats for the
+ * intermediate and the arrays' prescale exponents; in place allowed. */
+#define BDX_DENSE_WORK_FLOATS(count, h, w) ((size_t)(count) * (size_t)(h) * (size_t)(w) + (size_t)(count))
 int bdx_dense_transform(const float *d_in, float *d_out, float *d_work, int h, int w, size_t batch, int mode,
                         int prec, const float *d_qh, const float *d_qht, const float *d_qw, float scale,
                         void *stream);
 
 /* The same for `count` h x w arrays at the element offsets d_offsets[0, count) of d_in and
  * d_out (the arrays of one shape inside a ragged batch); count <= 65535 and
- * count h / 64 <= 65535; d_work: count h w floats, contiguous. */
+ * count h / 64 <= 65535; d_work: BDX_DENSE_WORK_FLOATS(count, h, w) floats, contiguous. */
 int bdx_dense_transform_gather(const float *d_in, float *d_out, float *d_work, int h, int w, size_t count,
                                const size_t *d_offsets, int mode, int prec, const float *d_qh, const float *d_qht,
                                const float *d_qw, float scale, void *stream);

@@ -311,7 +311,10 @@ int bridct_plan_get_pack(const bridct_plan *plan)
  * plan's own workspace (grown on demand) when work is NULL. */
 static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, size_t count, float *work)
 {
-    const size_t need = count * plan->elements;
+    const size_t max_batch = (size_t)65535 * 64 / (size_t)plan->height < 65535
+                                 ? (size_t)65535 * 64 / (size_t)plan->height : 65535;
+    const size_t slice = count < max_batch ? count : max_batch;
+    const size_t need = BDX_DENSE_WORK_FLOATS(slice, plan->height, plan->width);
     if (!work && need > plan->work_floats) {
         if (plan->d_work && (bdx_stream_sync(plan->stream) || bdx_free(plan->d_work)))
             return -1;
@@ -326,8 +329,6 @@ static int dense(bridct_plan *plan, int mode, const float *d_in, float *d_out, s
     const size_t hh = (size_t)plan->height * plan->height;
     const int prec = plan->route == BRIDCT_ROUTE_MFMA_F16 ? BDX_DENSE_F16 : BDX_DENSE_BF16;
     /* batches beyond one launch's grid limits go in slices */
-    const size_t max_batch = (size_t)65535 * 64 / (size_t)plan->height < 65535
-                                 ? (size_t)65535 * 64 / (size_t)plan->height : 65535;
     for (size_t done = 0; done < count;) {
         const size_t n = count - done < max_batch ? count - done : max_batch;
         const size_t off = done * plan->elements;
@@ -514,7 +515,8 @@ int bridct_plan_apply_device_static(bridct_plan *plan, enum bridct_mode mode, co
         size_t floats = 0;
         if (bdx_dense_static_work(plan->device, &work, &floats))
             return BRIDCT_DEVICE_ERROR;
-        if (batch * plan->elements > floats)
+        if (batch * plan->elements > BDX_STATIC_ARRAY_FLOATS ||
+            BDX_DENSE_WORK_FLOATS(batch, plan->height, plan->width) > floats)
             return BRIDCT_WORKSPACE_TOO_SMALL;
     }
     if (transform_with(plan, (int)mode, d_input, d_output, batch, work))

@@ -29,10 +29,10 @@ BRIDCT_API int bridct_plan_apply_device(bridct_plan *plan, enum bridct_mode mode
  * callers that cannot allocate device memory during the call (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation).  The butterfly route
  * needs no memory beyond the plan's; a dense-route plan (bridct_plan_set_route) takes its
- * intermediate from one static array per GPU of 8,388,608 floats (32 MiB, held by each GPU
- * the library runs on until the process ends) instead of allocating its own buffer, so
- * batch * height * width may not exceed 8,388,608 there (else BRIDCT_WORKSPACE_TOO_SMALL,
- * nothing run).  Every such call on one GPU shares that array: dense-route calls that can
+ * intermediate (and its per-array prescale exponents) from one static array per GPU of
+ * 8,454,144 floats (32.25 MiB, held by each GPU the library runs on until the process
+ * ends) instead of allocating its own buffer, so batch * height * width may not exceed
+ * 8,388,608 there (else BRIDCT_WORKSPACE_TOO_SMALL, nothing run).  Every such call on one GPU shares that array: dense-route calls that can
  * overlap on one GPU (other streams, other threads) race and must be serialized by the
  * caller.  Results equal bridct_plan_apply_device's bitwise. */
 BRIDCT_API int bridct_plan_apply_device_static(bridct_plan *plan, enum bridct_mode mode,
@@ -42,10 +42,11 @@ BRIDCT_API int bridct_plan_apply_device_static(bridct_plan *plan, enum bridct_mo
  * MFMA_F16 / MFMA_BF16: the dense route Y = C_h X C_w^T on the matrix cores, both
  * operands split into hi + lo half-precision parts (FP16 or BF16), three products, FP32
  * accumulation; both axes must be at least 64.  Its error against the FP64 definition is
- * larger than the butterflies' (measured per shape); the FP16 split needs inputs of
- * moderate range (|x| up to about 30 and not uniformly tiny, or half-precision overflow
- * and underflow of the split parts degrade it).  The dense route keeps an intermediate
- * buffer of batch h w floats in the plan, allocated on the first call that needs it. */
+ * larger than the butterflies' (measured per shape).  Each array is prescaled by the power
+ * of two of its largest magnitude before the split and scaled back after, so the error
+ * does not depend on the array's scale (2^k X gives exactly 2^k times the result of X).
+ * The dense route keeps an intermediate buffer (one slice of batch h w floats plus one
+ * exponent per array) in the plan, allocated on the first call that needs it. */
 #define BRIDCT_ROUTE_BUTTERFLY 0
 #define BRIDCT_ROUTE_MFMA_F16 1
 #define BRIDCT_ROUTE_MFMA_BF16 2

@@ -326,9 +326,7 @@ def test_grouped_follows_table_routes(tmp_path, monkeypatch):
     a grouped batch runs the same dense GEMMs in gather form and equals the per-shape plans
     bitwise (eager and graph replay, every mode), the automatic strategy is GROUPED, and
     the persistent kernel, which runs butterflies only, agrees with butterfly plans to FP32
-    rounding.  A dense entry for a shape with an axis below 64 is ignored by both.  (The
-    split routes' own error grows for arrays far below unit scale, where the FP16 lo halves
-    become subnormal; the dense tests cover it.)"""
+    rounding.  A dense entry for a shape with an axis below 64 is ignored by both."""
     table = tmp_path / 'schedule.txt'
     table.write_text('64 64 mfma_f16 0\n128 256 mfma_f16 0\n64 1024 mfma_bf16 0\n512 64 mfma_f16 0\n'
                      '32 64 mfma_f16 0\n')

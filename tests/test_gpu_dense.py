@@ -1,6 +1,8 @@
 """GPU tests of the dense split-MFMA route (Y = C_h X C_w^T on the matrix cores from FP16
 or BF16 hi + lo operands, three products, FP32 accumulation): error against the FP64
-oracle per shape and route, written to gpurun_out/dense_errors.json, and the route's API
+oracle for every shape with both axes >= 64, both directions, at scales 1, 1e-3 and 1e3,
+with the butterfly route's error on the same arrays (written to
+gpurun_out/dense_errors.json), exact power-of-two scale equivariance, and the route's API
 rules.  The reference has no dense route; its definition of the transform is
 docs/API.md:7-13 (the oracle's Q matrices)."""
 import json
@@ -17,7 +19,10 @@ pytestmark = pytest.mark.gpu
 # Error bounds of the split routes, written from the precision of the split: a hi + lo
 # pair carries about 22 bits (FP16) or 16 bits (BF16) of each operand, accumulated in FP32.
 BOUND = {capi.ROUTE_MFMA_F16: 2e-6, capi.ROUTE_MFMA_BF16: 6e-5}
-SHAPES = [(64, 64), (256, 256), (1024, 1024), (64, 256), (256, 64), (128, 1024), (512, 128)]
+NAMES = {capi.ROUTE_MFMA_F16: 'f16', capi.ROUTE_MFMA_BF16: 'bf16'}
+# every shape with both axes >= 64 (the route's domain)
+SHAPES = [(h, w) for h in (64, 128, 256, 512, 1024) for w in (64, 128, 256, 512, 1024)]
+SCALES = (1.0, 1e-3, 1e3)
 ERRORS = {}
 
 
@@ -32,29 +37,67 @@ def _run(plan, mode, x):
     return y
 
 
-@pytest.mark.parametrize('route', [capi.ROUTE_MFMA_F16, capi.ROUTE_MFMA_BF16])
+def _errors(y, ref):
+    worst = float(max(S.max_rel(y[b], ref[b]) for b in range(y.shape[0])))
+    rms = float(np.sqrt(np.mean((y - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    return worst, rms
+
+
 @pytest.mark.parametrize('h,w', SHAPES)
-def test_dense_route_error(route, h, w):
+def test_dense_route_error(h, w):
+    """Both split routes, forward and inverse, at unit scale and at 1e-3 and 1e3: error
+    against FP64 within the route's bound, stated beside the butterfly route's on the same
+    arrays; and the per-array power-of-two prescale makes the route scale-equivariant
+    (2^k X gives exactly 2^k times the result)."""
     batch = 2 if h * w >= 1 << 20 else 4
-    x = np.random.default_rng(h * 7 + w).uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    base = np.random.default_rng(h * 7 + w).uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    plans = {r: capi.Plan(h, w) for r in (capi.ROUTE_BUTTERFLY, capi.ROUTE_MFMA_F16, capi.ROUTE_MFMA_BF16)}
+    try:
+        for r, p in plans.items():
+            assert p.set_route(r) == 0 and p.route == r
+        print('dense route %dx%d' % (h, w), flush=True)
+        for scale in SCALES:
+            x = (base * np.float32(scale)).astype(np.float32)
+            for mode in (capi.FORWARD, capi.INVERSE):
+                ref = S.oracle_2d(x, mode)
+                bf = _errors(_run(plans[capi.ROUTE_BUTTERFLY], mode, x), ref)
+                for route in (capi.ROUTE_MFMA_F16, capi.ROUTE_MFMA_BF16):
+                    err, rms = _errors(_run(plans[route], mode, x), ref)
+                    key = '%s %dx%d %s scale %g' % (NAMES[route], h, w, 'fwd' if mode == 0 else 'inv', scale)
+                    ERRORS[key] = {'max_rel': err, 'rms_rel': rms, 'butterfly_max_rel': bf[0],
+                                   'butterfly_rms_rel': bf[1], 'ratio_to_butterfly': err / bf[0]}
+                    assert err < BOUND[route], (key, err)
+        # exact scale equivariance through the power-of-two prescale
+        for route in (capi.ROUTE_MFMA_F16, capi.ROUTE_MFMA_BF16):
+            y0 = _run(plans[route], capi.FORWARD, base)
+            for k in (-20, 17):
+                yk = _run(plans[route], capi.FORWARD, np.ldexp(base, k).astype(np.float32))
+                assert np.array_equal(yk, np.ldexp(y0, k).astype(np.float32)), (route, k)
+        back = _run(plans[capi.ROUTE_MFMA_F16], capi.ROUNDTRIP, base)
+        assert S.max_rel(back, base) < 3 * BOUND[capi.ROUTE_MFMA_F16]
+    finally:
+        for p in plans.values():
+            p.close()
+
+
+def test_dense_route_tiny_and_mixed_scale_arrays():
+    """One batch whose arrays span 10^-30 .. 10^30 (and an all-zero array): each array's
+    result is within the bound of its own scale; zeros stay exact zeros."""
+    h = w = 64
+    rng = np.random.default_rng(12)
+    scales = [1e-30, 1e-6, 1.0, 1e6, 1e30, 0.0]
+    x = np.stack([rng.uniform(-1, 1, (h, w)) * s for s in scales]).astype(np.float32)
     plan = capi.Plan(h, w)
     try:
-        assert plan.set_route(route) == 0 and plan.route == route
-        print('dense route', route, h, w, flush=True)
-        rec = {}
+        assert plan.set_route(capi.ROUTE_MFMA_F16) == 0
         for mode in (capi.FORWARD, capi.INVERSE):
             y = _run(plan, mode, x)
             ref = S.oracle_2d(x, mode)
-            err = float(max(S.max_rel(y[b], ref[b]) for b in range(batch)))
-            rms = float(np.sqrt(np.mean((y - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
-            rec['fwd' if mode == 0 else 'inv'] = {'max_rel': err, 'rms_rel': rms}
-            assert err < BOUND[route], (mode, err)
-        back = _run(plan, capi.ROUNDTRIP, x)
-        rec['roundtrip_max_rel'] = float(S.max_rel(back, x))
-        # forward then inverse: both errors, the forward one relative to max|Y| (about three
-        # times max|x| for these inputs)
-        assert rec['roundtrip_max_rel'] < 3 * BOUND[route]
-        ERRORS['%s %dx%d' % ('f16' if route == capi.ROUTE_MFMA_F16 else 'bf16', h, w)] = rec
+            for i, s in enumerate(scales):
+                if s == 0.0:
+                    assert not np.any(y[i])
+                else:
+                    assert S.max_rel(y[i], ref[i]) < BOUND[capi.ROUTE_MFMA_F16], (mode, s)
     finally:
         plan.close()
 
