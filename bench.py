@@ -5,7 +5,8 @@ Workload (the large and rectangular plain-FP32 configuration): batches of 64 flo
 arrays at 1024x1024, 512x128, 128x1024 and 32x256, values a smooth field (sum of 16
 Gaussian blobs, amplitude renormalized into [-1, 1]) plus N(0, 0.01^2) noise.  One step
 is the orthonormal forward DCT-II and then the inverse DCT-III of every batch, with inputs
-already resident in HBM.  Prints one JSON line: whole-job Gpixel/s of forward + inverse,
+already resident in HBM.  Before the W warm-up steps the step runs for about 0.3 s so that the
+timed steps see the GPU's settled clock.  Prints one JSON line: whole-job Gpixel/s of forward + inverse,
 per-shape forward / inverse rates, the HBM roofline of the dominant kernel (each kernel of
 the 64 x 1024^2 transforms timed in sequence, row pass then column pass, with HIP events on
 the plan's stream), device copies as the practical bandwidth reference, and the reference
@@ -104,9 +105,45 @@ def smooth_field(h, w, batch, rng):
     return out
 
 
-def cpu_baseline(inputs):
-    """Reference CPU library (oracle/_ref, built from the reference sources) timed on this
-    host: a bounded sample of the same workload, forward then inverse, all host cores."""
+def host_cpus():
+    """CPU model, physical cores, logical CPUs, the CPUs this process may run on (affinity)
+    and its cgroup CPU quota; `threads` is the smallest of physical cores, affinity and
+    quota: more threads than that only time-slice."""
+    model, cores = 'unknown', set()
+    try:
+        phys = core = None
+        for line in open('/proc/cpuinfo'):
+            key, _, val = line.partition(':')
+            key, val = key.strip(), val.strip()
+            if key == 'model name' and model == 'unknown':
+                model = val
+            elif key == 'physical id':
+                phys = val
+            elif key == 'core id':
+                core = val
+            elif not key and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    physical = len(cores) or logical
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else logical
+    quota = None
+    try:
+        q, period = open('/sys/fs/cgroup/cpu.max').read().split()
+        if q != 'max':
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = min(physical, affinity, max(1, int(quota)) if quota else physical, 256)
+    return {'model': model, 'physical_cores': physical, 'logical_cpus': logical, 'affinity_cpus': affinity,
+            'cgroup_cpu_quota': quota, 'threads': threads}
+
+
+def _ref_bench():
     path = os.path.join(ROOT, 'oracle', '_ref', 'libref_bench.so')
     if not os.path.exists(path):
         return None
@@ -116,35 +153,52 @@ def cpu_baseline(inputs):
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.POINTER(ctypes.c_double)]
-    per_shape = 8
-    hs, ws, offs, chunks = [], [], [], []
-    off = 0
-    for (h, w), x in inputs.items():
-        for i in range(per_shape):
-            hs.append(h)
-            ws.append(w)
+    return lib
+
+
+def _ref_rate(lib, shapes, offsets, data, threads, blocks=5, target_ms=200.0):
+    """Reference CPU forward-then-inverse (materialized spectrum) over arrays `shapes` at
+    `offsets` of `data`: Gpixel/s of the median pass, or None."""
+    h_a = np.ascontiguousarray([h for h, _ in shapes], np.int32)
+    w_a = np.ascontiguousarray([w for _, w in shapes], np.int32)
+    o_a = np.ascontiguousarray(offsets, np.uint64)
+    out, spec = np.empty_like(data), np.empty_like(data)
+    ns = ctypes.c_double()
+    if lib.ref_bench_run(len(shapes), h_a.ctypes.data, w_a.ctypes.data, o_a.ctypes.data, data.ctypes.data,
+                         out.ctypes.data, spec.ctypes.data, 3, threads, blocks, target_ms, ctypes.byref(ns)):
+        return None
+    return round(data.size / ns.value, 4)
+
+
+def cpu_baseline(inputs, blocks=None, ragged=None):
+    """Reference CPU library (oracle/_ref, built from the reference sources) timed on this
+    host, one plan per thread and static partitions (the bench/bench.c contract: plans and
+    buffers made first, warm-up, repetitions doubled to >= 200 ms, median of 5 blocks), on
+    as many threads as the process has CPUs (physical cores, capped by affinity and the
+    cgroup quota).  `value`: the headline workload's four shapes, forward then inverse, one
+    array of each shape per thread (equal work per thread).  Also: the 256x256 method (C1)
+    on 1 and all threads, the 1,048,576 8x8 blocks (C4) and the 10,000-array ragged mix
+    (C5), each forward then inverse through per-array plan calls, and the reference's own
+    forward FP64 error per shape class on the ragged sample the GPU leg states."""
+    lib = _ref_bench()
+    if lib is None:
+        return None
+    cpus = host_cpus()
+    threads = cpus['threads']
+    # headline sample: arrays interleaved by shape, so that each thread's contiguous
+    # partition holds one array of every shape
+    shapes, offs, chunks, off = [], [], [], 0
+    for i in range(threads):
+        for (h, w), x in inputs.items():
+            shapes.append((h, w))
             offs.append(off)
-            chunks.append(x[i].ravel())
+            chunks.append(x[i % x.shape[0]].ravel())
             off += h * w
     data = np.concatenate(chunks).astype(np.float32)
-    out = np.empty_like(data)
-    spec = np.empty_like(data)
-    threads = min(os.cpu_count() or 1, 16)
-    h_a = np.array(hs, np.int32)
-    w_a = np.array(ws, np.int32)
-    o_a = np.array(offs, np.uint64)
-    ns = ctypes.c_double()
-    rc = lib.ref_bench_run(len(hs), h_a.ctypes.data, w_a.ctypes.data, o_a.ctypes.data,
-                           data.ctypes.data, out.ctypes.data, spec.ctypes.data, 3, threads, 5,
-                           200.0, ctypes.byref(ns))
-    if rc:
+    gpx = _ref_rate(lib, shapes, offs, data, threads)
+    if gpx is None:
         return None
-    gpx = data.size / ns.value
-    ns_1 = ctypes.c_double()
-    gpx_1 = None
-    if lib.ref_bench_run(len(hs), h_a.ctypes.data, w_a.ctypes.data, o_a.ctypes.data, data.ctypes.data,
-                         out.ctypes.data, spec.ctypes.data, 3, 1, 5, 200.0, ctypes.byref(ns_1)) == 0:
-        gpx_1 = round(data.size / ns_1.value, 4)
+    gpx_1 = _ref_rate(lib, shapes[:len(inputs)], offs[:len(inputs)], data[:offs[len(inputs) - 1] + shapes[len(inputs) - 1][0] * shapes[len(inputs) - 1][1]], 1)
     lib.ref_bench_avx2_fma.restype = ctypes.c_int
     avx2 = bool(lib.ref_bench_avx2_fma())
     ref = ctypes.CDLL(os.path.join(ROOT, 'oracle', '_ref', 'libbridct.so'))
@@ -153,6 +207,7 @@ def cpu_baseline(inputs):
     ref.bridct_plan_route.restype = ctypes.c_char_p
     ref.bridct_plan_route.argtypes = [ctypes.c_void_p]
     ref.bridct_plan_destroy.argtypes = [ctypes.c_void_p]
+    ref.bridct_plan_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     routes = {}
     for h, w in list(inputs) + [(256, 256)]:
         p = ref.bridct_plan_create(h, w)
@@ -160,29 +215,56 @@ def cpu_baseline(inputs):
         ref.bridct_plan_destroy(p)
     # the plain 256x256 method: 256 arrays (seeds 0..255), forward then inverse
     c1 = c1_inputs()
-    n1 = c1.shape[0]
-    c1_h = np.full(n1, 256, np.int32)
-    c1_o = (np.arange(n1, dtype=np.uint64) * 65536)
-    c1_out, c1_spec = np.empty_like(c1), np.empty_like(c1)
-    c1_rates = {}
-    for t in (1, threads):
-        ns1 = ctypes.c_double()
-        if lib.ref_bench_run(n1, c1_h.ctypes.data, c1_h.ctypes.data, c1_o.ctypes.data, c1.ctypes.data,
-                             c1_out.ctypes.data, c1_spec.ctypes.data, 3, t, 5, 200.0, ctypes.byref(ns1)) == 0:
-            c1_rates['threads_%d' % t] = round(c1.size / ns1.value, 4)
-    cpu = 'unknown'
-    try:
-        for line in open('/proc/cpuinfo'):
-            if line.startswith('model name'):
-                cpu = line.split(':', 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {'value': round(gpx, 4), 'unit': 'Gpixel/s', 'cores': threads, 'kind': 'reference',
-            'sample': '%d arrays of each shape (%d Mpx), forward then inverse, one plan per '
-                      'thread, median of 5 blocks; host CPU %s' % (per_shape, data.size // 10 ** 6, cpu),
-            'plain_256x256_gpx_s': c1_rates, 'value_threads_1': gpx_1, 'host_avx2_fma': avx2,
-            'reference_routes': routes}
+    c1_offs = [i * 65536 for i in range(c1.shape[0])]
+    c1_rates = {'threads_%d' % t: _ref_rate(lib, [(256, 256)] * c1.shape[0], c1_offs, c1.ravel(), t)
+                for t in sorted({1, threads})}
+    res = {'value': gpx, 'unit': 'Gpixel/s', 'cores': threads, 'kind': 'reference',
+           'sample': '%d arrays: one of each headline shape (1024x1024, 512x128, 128x1024, 32x256) per thread '
+                     '(%d Mpx), forward then inverse, one plan per thread, median of 5 blocks' % (
+                         len(shapes), data.size // 10 ** 6),
+           'host': cpus, 'value_threads_1': gpx_1, 'plain_256x256_gpx_s': c1_rates, 'host_avx2_fma': avx2,
+           'reference_routes': routes}
+    if blocks is not None:
+        nb = blocks.size // 64
+        res['blocks_8x8'] = {'blocks': nb, 'fwd_then_inv_gpx_s': _ref_rate(
+            lib, [(8, 8)] * nb, np.arange(nb, dtype=np.uint64) * 64, blocks, threads, 3)}
+    if ragged is not None:
+        x, rshapes, roffs, sample = ragged
+        res['ragged'] = {'arrays': len(rshapes), 'fwd_then_inv_gpx_s': _ref_rate(lib, rshapes, roffs, x, threads, 3)}
+        # the reference's own forward error on the arrays the GPU leg states its error on
+        per_class = {}
+        for i in sample:
+            (h, w), o = rshapes[i], roffs[i]
+            a = np.ascontiguousarray(x[o:o + h * w].reshape(h, w))
+            y = np.empty_like(a)
+            p = ref.bridct_plan_create(h, w)
+            ref.bridct_plan_apply(p, 0, a.ctypes.data, y.ctypes.data)
+            ref.bridct_plan_destroy(p)
+            _class_error(per_class, h, w, y, a)
+        res['ragged']['ref_forward_error_per_class'] = _class_summary(per_class)
+    return res
+
+
+def shape_class(h, w):
+    """Shape classes of the ragged error statement, by the longer axis."""
+    m = max(h, w)
+    return 'max_axis_8_32' if m <= 32 else 'max_axis_64_256' if m <= 256 else 'max_axis_512_1024'
+
+
+def _class_error(acc, h, w, y, x):
+    """Accumulates the forward error of y against the FP64 transform of x into acc[class]."""
+    ref = _dct_fp64(h) @ x.astype(np.float64) @ _dct_fp64(w).T
+    d = y.astype(np.float64) - ref
+    c = acc.setdefault(shape_class(h, w), {'arrays': 0, 'max_rel': 0.0, 'sq_err': 0.0, 'sq_ref': 0.0})
+    c['arrays'] += 1
+    c['max_rel'] = max(c['max_rel'], float(np.abs(d).max() / np.abs(ref).max()))
+    c['sq_err'] += float((d * d).sum())
+    c['sq_ref'] += float((ref * ref).sum())
+
+
+def _class_summary(acc):
+    return {k: {'arrays': v['arrays'], 'max_rel': v['max_rel'],
+                'rms_rel': (v['sq_err'] / v['sq_ref']) ** 0.5 if v['sq_ref'] else 0.0} for k, v in sorted(acc.items())}
 
 
 KERNELS_1024 = (
@@ -253,7 +335,7 @@ def leg_blocks(hip, lib):
     res['persistent_vs_per_block'] = round(res['fwd']['gpx_s'] / per_block, 1)
     plan.close()
     batch.close()
-    return res
+    return res, x
 
 
 def c1_inputs():
@@ -469,6 +551,23 @@ def leg_ragged(hip, lib):
     best = res['grouped' if batch.strategy == capi.BATCH_GROUPED else 'persistent']
     res['fwd'], res['inv'] = best['fwd'], best['inv']
     res['roundtrip_max_rel_err'] = worst
+    # forward error against FP64 per shape class, the automatic strategy, on up to three
+    # arrays of every distinct shape (max-relative per array; RMS over the class)
+    assert batch.set_strategy(capi.BATCH_AUTO) == 0
+    assert batch.apply_device(capi.FORWARD, da.ptr, db.ptr) == 0 and batch.synchronize() == 0
+    y = np.empty_like(x)
+    db.download(y)
+    seen, sample = {}, []
+    for i, s in enumerate(shapes):
+        if seen.get(s, 0) < 3:
+            seen[s] = seen.get(s, 0) + 1
+            sample.append(i)
+    per_class = {}
+    for i in sample:
+        (h, w), o = shapes[i], batch.offsets[i]
+        _class_error(per_class, h, w, y[o:o + h * w].reshape(h, w), x[o:o + h * w].reshape(h, w))
+    res['forward_error_per_class'] = _class_summary(per_class)
+    res['forward_error_sample'] = 'up to 3 arrays of each of the 64 shapes (%d arrays)' % len(sample)
     plans = {s: capi.Plan(*s) for s in sorted(set(shapes))}
     for name, mode in (('fwd', capi.FORWARD), ('inv', capi.INVERSE)):
         hip.sync()
@@ -484,8 +583,9 @@ def leg_ragged(hip, lib):
                                        plain, 2)
     for p in plans.values():
         p.close()
+    offsets = list(batch.offsets)
     batch.close()
-    return res
+    return res, (x, shapes, offsets, sample)
 
 
 def main():
@@ -526,6 +626,14 @@ def main():
             assert plan.apply_device(capi.FORWARD, a.ptr, b.ptr, BATCH) == 0
             assert plan.apply_device(capi.INVERSE, b.ptr, c.ptr, BATCH) == 0
 
+    # Setup: the step runs for about 0.3 s before the W warm-up steps, so that the timed
+    # steps see the clock the GPU settles at under this load (the first steps after idle
+    # run up to 10% slower, and 50 steps last only 25 ms).  Not part of the timed region.
+    t_ramp = time.perf_counter()
+    while time.perf_counter() - t_ramp < 0.3:
+        for _ in range(20):
+            step()
+        hip.sync()
     for _ in range(args.warmup):
         step()
     hip.sync()
@@ -601,14 +709,15 @@ def main():
         line['plain_256x256'] = leg_c1(hip, lib)
     if 'dense' in legs:
         line['dense_route'] = leg_dense(hip)
+    blocks_x = ragged_data = None
     if 'blocks' in legs:
-        line['blocks_8x8'] = leg_blocks(hip, lib)
+        line['blocks_8x8'], blocks_x = leg_blocks(hip, lib)
     if 'ragged' in legs:
-        line['ragged'] = leg_ragged(hip, lib)
+        line['ragged'], ragged_data = leg_ragged(hip, lib)
     if 'codec' in legs:
         line['codec_8x8'] = leg_codec(hip)
     if not args.no_cpu:
-        line['cpu_baseline'] = cpu_baseline(inputs)
+        line['cpu_baseline'] = cpu_baseline(inputs, blocks_x, ragged_data)
     print(json.dumps(line))
     return 0
 
