@@ -17,11 +17,19 @@
 namespace bdx {
 namespace {
 
-// One wave per team member, capped at 512 lanes: 1024-lane blocks would cap a lane at 128
-// VGPRs, which the length-1024 column team exceeds (scratch spills).
-template <int H> constexpr int col_threads()
+// Lanes per column-pass workgroup: one wave per team member up to the cap (uniform member
+// index: scalar row addresses and constants), else two members per wave.  Measured on
+// 64 x 1024^2 (length-1024 team, 16 members):
+//   forward: 512 lanes (two members per wave, 103 VGPRs, two workgroups per CU) 99.7 us,
+//            1024 lanes (one member per wave, 97 VGPRs, one workgroup per CU) 115.8 us;
+//   inverse: 512 lanes 128.7 us -- the self-paired group g = 0 is a divergent branch in
+//            wave 0 whose results stay in registers, so both branches' values are live at
+//            once (150 VGPRs, one workgroup per CU) -- 1024 lanes 116.0 us (126 VGPRs).
+// The fused round trip keeps 512 lanes: at 1024 its length-1024 team spills.
+template <int H, int MODE> constexpr int col_threads()
 {
-    return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
+    constexpr int cap = MODE == 1 ? 1024 : 512;
+    return H >= 128 ? (64 * team_size<H>() < cap ? 64 * team_size<H>() : cap) : kThreads;
 }
 
 template <int W, int OP, bool G, bool NT>
@@ -65,11 +73,11 @@ __device__ __forceinline__ void cols_body(const float *in, float *out, int w, co
 }
 
 template <int H, int MODE, bool G, bool NT>
-__global__ __launch_bounds__(col_threads<H>()) void k_cols(const float *in, float *out, int w,
-                                                            const size_t *__restrict__ offs,
-                                                            const float *__restrict__ tab)
+__global__ __launch_bounds__((col_threads<H, MODE>())) void k_cols(const float *in, float *out, int w,
+                                                                  const size_t *__restrict__ offs,
+                                                                  const float *__restrict__ tab)
 {
-    cols_body<H, MODE, G, NT, col_threads<H>()>(in, out, w, offs, tab);
+    cols_body<H, MODE, G, NT, col_threads<H, MODE>()>(in, out, w, offs, tab);
 }
 
 // The same column pass with narrower strips (TH = col_threads / 2 or / 4: 32 or 16
@@ -132,11 +140,10 @@ int cols_narrow(int w, const float *in, float *out, size_t batch, int mode, cons
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int H, bool G, bool NT>
-int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
-            const size_t *offs)
+template <int H, int MODE, bool G, bool NT>
+int cols_mode(int w, const float *in, float *out, size_t batch, const float *tab, hipStream_t s, const size_t *offs)
 {
-    constexpr int TH = col_threads<H>(), L = col_unit_cols<H, TH>();
+    constexpr int TH = col_threads<H, MODE>(), L = col_unit_cols<H, TH>();
     if (w % L)
         return -1;
     // Small grids (never streaming ones: those exceed the 256 MB cache) take strips of half
@@ -147,21 +154,28 @@ int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const fl
         if (batch * (size_t)(w / L) < kColsMinGrid) {
             if constexpr (L4 >= 16) {
                 if (batch * (size_t)(w / L2) >= kColsMinGrid)
-                    return cols_narrow<H, G, TH / 2>(w, in, out, batch, mode, tab, s, offs);
-                return cols_narrow<H, G, TH / 4>(w, in, out, batch, mode, tab, s, offs);
+                    return cols_narrow<H, G, TH / 2>(w, in, out, batch, MODE, tab, s, offs);
+                return cols_narrow<H, G, TH / 4>(w, in, out, batch, MODE, tab, s, offs);
             } else if constexpr (L2 >= 16) {
-                return cols_narrow<H, G, TH / 2>(w, in, out, batch, mode, tab, s, offs);
+                return cols_narrow<H, G, TH / 2>(w, in, out, batch, MODE, tab, s, offs);
             }
         }
     }
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
+    hipLaunchKernelGGL((k_cols<H, MODE, G, NT>), grid, block, 0, s, in, out, w, offs, tab);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int H, bool G, bool NT>
+int cols_hg(int w, const float *in, float *out, size_t batch, int mode, const float *tab, hipStream_t s,
+            const size_t *offs)
+{
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_cols<H, 0, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 1: hipLaunchKernelGGL((k_cols<H, 1, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 2: hipLaunchKernelGGL((k_cols<H, 2, G, NT>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 0: return cols_mode<H, 0, G, NT>(w, in, out, batch, tab, s, offs);
+    case 1: return cols_mode<H, 1, G, NT>(w, in, out, batch, tab, s, offs);
+    case 2: return cols_mode<H, 2, G, NT>(w, in, out, batch, tab, s, offs);
     default: return -1;
     }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int H>

@@ -150,54 +150,47 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int t = M == 1 ? __builtin_amdgcn_readfirstlane(wave) : wave * M + lane / L;
         const int line = lane % L;
-        const float *src = in + base + c0;
-        float *dst = out + base + c0;
         const bool live = line < ncols;
-        // Byte offsets from the strip base in 32 bits (an array is at most 4 MiB): the loads
-        // and stores then take the uniform base plus a 32-bit lane offset, with no 64-bit
-        // address add per access (length 1024: 23% fewer non-arithmetic vector instructions,
-        // 140 / 149 VGPRs instead of 142 / 154, and the round trip no longer spills).  The
-        // length-512 round trip keeps element offsets: in this form its registers rose from
-        // 137 to 252 and it ran 4% slower.  Equal addresses either way.
-        constexpr bool kByteOff = !(H == 512 && MODE == 2);
-        const auto ld = [&](int i) {
-            if constexpr (kByteOff)
-                return live ? ld1<NT>((const float *)((const char *)src + (unsigned)(i * w + line) * 4u)) : 0.0f;
-            else
-                return live ? ld1<NT>(src + (unsigned)(i * w) + line) : 0.0f;
+        // Row r of the strip is at the uniform strip base + r w floats.  The lane keeps one
+        // byte offset per row family (lane part of the row, plus its column); each access
+        // adds a uniform row offset in scalar registers (an array is at most 4 MiB, so the
+        // offsets fit 32 bits).
+        const unsigned w4 = 4u * (unsigned)w, l4 = 4u * (unsigned)line;
+        const int row_p = 4 * t, row_r = 4 * (T - 1 - t), row_c = t, row_d = T - 1 - t;
+        const auto lane_row = [=](int fam) {
+            return fam == FAM_P ? row_p : fam == FAM_R ? row_r : fam == FAM_C ? row_c : row_d;
         };
-        const auto st = [&](int i, float v) {
-            if (!live)
-                return;
-            if constexpr (kByteOff)
-                st1<NT>((float *)((char *)dst + (unsigned)(i * w + line) * 4u), v);
-            else
-                st1<NT>(dst + (unsigned)(i * w) + line, v);
+        const unsigned off_p = (unsigned)row_p * w4 + l4, off_r = (unsigned)row_r * w4 + l4;
+        const unsigned off_c = (unsigned)row_c * w4 + l4, off_d = (unsigned)row_d * w4 + l4;
+        const auto off = [=](int fam) {
+            return fam == FAM_P ? off_p : fam == FAM_R ? off_r : fam == FAM_C ? off_c : off_d;
+        };
+        const char *src = reinterpret_cast<const char *>(in + base + c0);
+        char *dst = reinterpret_cast<char *>(out + base + c0);
+        const auto ld = [&](int fam, int urow) {
+            return live ? ld1<NT>(reinterpret_cast<const float *>(src + (size_t)urow * w4 + off(fam))) : 0.0f;
+        };
+        const auto st = [&](int fam, int urow, float v) {
+            if (live)
+                st1<NT>(reinterpret_cast<float *>(dst + (size_t)urow * w4 + off(fam)), v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
+        const auto alpha = [&](int fam, int urow) { return al[lane_row(fam) + urow]; };
         const CConst<H, M> cc{tab + BDX_TABLE_BASE(H), lt};
-        float o[Q][T][4];
         if constexpr (MODE == 1) {
-#pragma unroll
-            for (int r = 0; r < Q; r++)
-#pragma unroll
-                for (int k2 = 0; k2 < T; k2++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int idx = wteam_index<H, T>(t + T * r, k2, c);
-                        o[r][k2][c] = ld(idx) * al[idx];
-                    }
-            wteam_inv<H, THREADS>(t, line, o, st, xl, cc);
-        } else {
-            wteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int r, int k2, int c, float v) {
-                const int idx = wteam_index<H, T>(t + T * r, k2, c);
-                if constexpr (MODE == 0)
-                    st(idx, v * al[idx]);
-                else
-                    o[r][k2][c] = v * al[idx];
+            cteam_inv<H, THREADS>(
+                t, line, [&](int, int, int, int fam, int urow) { return ld(fam, urow) * alpha(fam, urow); }, st, xl,
+                cc);
+        } else if constexpr (MODE == 0) {
+            cteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int, int, int, int fam, int urow, float v) {
+                st(fam, urow, v * alpha(fam, urow));
             });
-            if constexpr (MODE == 2)
-                wteam_inv<H, THREADS>(t, line, o, st, xl, cc);
+        } else {
+            float o[Q][T][4];
+            cteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int r, int k2, int c, int fam, int urow, float v) {
+                o[r][k2][c] = v * alpha(fam, urow);
+            });
+            cteam_inv<H, THREADS>(t, line, [&](int r, int k2, int c, int, int) { return o[r][k2][c]; }, st, xl, cc);
         }
     } else {
         // one lane per column: the generated Shao-Johnson core on H values in registers

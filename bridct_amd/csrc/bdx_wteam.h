@@ -10,6 +10,20 @@
 // are 256-byte (M = 1) or 128-byte (M = 2) row segments.  The exchange between members
 // goes through LDS with block barriers, in ROUNDS slices of the lines to bound the LDS
 // footprint.  Algorithm and index maps as in bdx_lines.h and bdx_post.h.
+//
+// Addressing.  Every row a member reads or writes is a lane part plus a wave-uniform part:
+// the Makhoul-packed inputs m = t + T j1 sit at rows 4t + 4T j1 (+2) in the first half and
+// 4(T-1-t) + (2n - 1 - 4T j1 - 4(T-1)) (-2) in the second (families P and R), and the
+// outputs of pair group g = t + T r at rows t + (T r + N1 k2), (T-1-t) + (n - T + 1 - T r -
+// N1 k2), (T-1-t) + (N - T + 1 - T r - N1 k2) and t + (N + T r + N1 k2) (families C, D, D,
+// C).  So a lane keeps four byte offsets (one per family) and each access adds a uniform
+// row offset: the address is a scalar base plus one of those four registers, with no
+// per-access vector arithmetic.  The member of lanes with t = 0 also owns the self-paired
+// group g = 0, whose rows are absolute (its lane part is 0 in family C).
+//
+// Registers.  The exchange writes a lane's N1 complex values to LDS and reads the values
+// of its pair groups back into the same registers (Q 2 T = N1), so one set of 2 N1 floats
+// is live across it, not two.
 #pragma once
 #include "bdx_lines.h"
 #include "bdx_post.h"
@@ -32,22 +46,6 @@ template <int n, int THREADS> struct CTeam {
     static_assert(M >= 1 && 64 % M == 0, "team members must tile the wave");
 };
 
-// Element index of output slot (k2, which) of pair group g.
-template <int n, int T>
-__device__ __forceinline__ int wteam_index(int g, int k2, int which)
-{
-    constexpr int N = n / 2, N1 = N / T;
-    int k;
-    if (g == 0) {
-        if (k2 == T / 2)
-            return which == 0 ? 0 : which == 1 ? N : which == 2 ? n / 4 : 3 * n / 4;
-        k = k2 < T / 2 ? N1 / 2 + N1 * k2 : N1 * (k2 - T / 2);
-    } else {
-        k = g + N1 * k2;
-    }
-    return which == 0 ? k : which == 1 ? n - k : which == 2 ? N - k : N + k;
-}
-
 // Constants of member t: from the global table block (M = 1, wave-uniform) or from the
 // LDS copy (M > 1).
 template <int n, int M> struct CConst {
@@ -66,22 +64,124 @@ template <int n, int M> struct CConst {
     __device__ __forceinline__ float tn(int k) const { return M == 1 ? g[2 * n + k] : l[RowTab<n>::TN + k]; }
 };
 
-// Forward core.  ld(i) returns element i of the lane's line; emit(r, k2, w, v) receives
-// the unscaled output of index wteam_index(t + T r, k2, w) as soon as it is computed.
+// Row families (lane parts): 0 = P (4t), 1 = R (4(T-1-t)), 2 = C (t), 3 = D (T-1-t).
+enum { FAM_P = 0, FAM_R = 1, FAM_C = 2, FAM_D = 3 };
+
+// Family and uniform part of output row (k2, which) of pair group g = t + T r, g != 0.
+template <int n, int T> __device__ __forceinline__ constexpr int out_family(int which)
+{
+    return which == 1 || which == 2 ? FAM_D : FAM_C;
+}
+template <int n, int T> __device__ __forceinline__ constexpr int out_urow(int r, int k2, int which)
+{
+    constexpr int N = n / 2, N1 = N / T;
+    return which == 0 ? T * r + N1 * k2
+         : which == 1 ? n - T + 1 - T * r - N1 * k2
+         : which == 2 ? N - T + 1 - T * r - N1 * k2
+                      : N + T * r + N1 * k2;
+}
+// Absolute output row (k2, which) of the self-paired group g = 0 (family C, lane part 0).
+template <int n, int T> __device__ __forceinline__ constexpr int out_row_g0(int k2, int which)
+{
+    constexpr int N = n / 2, N1 = N / T;
+    if (k2 == T / 2)
+        return which == 0 ? 0 : which == 1 ? N : which == 2 ? n / 4 : 3 * n / 4;
+    const int k = k2 < T / 2 ? N1 / 2 + N1 * k2 : N1 * (k2 - T / 2);
+    return which == 0 ? k : which == 1 ? n - k : which == 2 ? N - k : N + k;
+}
+// Family and uniform part of the packed input pair j1 (u: its first row; the second is u + 2
+// in family P, u - 2 in family R).
+template <int n, int T> __device__ __forceinline__ constexpr int in_family(int j1)
+{
+    return j1 < n / 2 / T / 2 ? FAM_P : FAM_R;
+}
+template <int n, int T> __device__ __forceinline__ constexpr int in_urow(int j1)
+{
+    return j1 < n / 2 / T / 2 ? 4 * T * j1 : 2 * n - 1 - 4 * T * j1 - 4 * (T - 1);
+}
+
+// Exchange: the lane's N1 complex values (k1 = 0..N1-1 at slot k1 T + t) go to LDS; the
+// values of its pair groups (ka = g or 0, kb = N1 - g or N1/2, j2 = 0..T-1) come back into
+// the same registers, dr[r][s][j2] = v[(2 r + s) T + j2].  TO_PAIRS: forward direction
+// (lane order in, pair order out); else the reverse.
+template <int n, int THREADS, bool TO_PAIRS>
+__device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTeam<n, THREADS>::N1],
+                                               float (&im)[CTeam<n, THREADS>::N1], float *xl)
+{
+    using C = CTeam<n, THREADS>;
+    constexpr int T = C::T, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
+    const int sl = line % SL, rnd = line / SL;
+    const auto lane_slot = [&](int k1) { return 2 * (k1 * T + t) * SL + sl; };
+    const auto pair_slot = [&](int r, int s, int j2) {
+        const int g = t + T * r;
+        const int k = s == 0 ? (g == 0 ? 0 : g) : (g == 0 ? N1 / 2 : N1 - g);
+        return 2 * (k * T + j2) * SL + sl;
+    };
+#pragma unroll
+    for (int rr = 0; rr < ROUNDS; rr++) {
+        if (rnd == rr) {
+            if constexpr (TO_PAIRS) {
+#pragma unroll
+                for (int k1 = 0; k1 < N1; k1++) {
+                    xl[lane_slot(k1)] = re[k1];
+                    xl[lane_slot(k1) + SL] = im[k1];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < Q; r++)
+#pragma unroll
+                    for (int s = 0; s < 2; s++)
+#pragma unroll
+                        for (int j2 = 0; j2 < T; j2++) {
+                            xl[pair_slot(r, s, j2)] = re[(2 * r + s) * T + j2];
+                            xl[pair_slot(r, s, j2) + SL] = im[(2 * r + s) * T + j2];
+                        }
+            }
+        }
+        __syncthreads();
+        if (rnd == rr) {
+            if constexpr (TO_PAIRS) {
+#pragma unroll
+                for (int r = 0; r < Q; r++)
+#pragma unroll
+                    for (int s = 0; s < 2; s++)
+#pragma unroll
+                        for (int j2 = 0; j2 < T; j2++) {
+                            re[(2 * r + s) * T + j2] = xl[pair_slot(r, s, j2)];
+                            im[(2 * r + s) * T + j2] = xl[pair_slot(r, s, j2) + SL];
+                        }
+            } else {
+#pragma unroll
+                for (int k1 = 0; k1 < N1; k1++) {
+                    re[k1] = xl[lane_slot(k1)];
+                    im[k1] = xl[lane_slot(k1) + SL];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int T> __device__ __forceinline__ float (&sub(float *v, int i))[T]
+{
+    return *reinterpret_cast<float(*)[T]>(v + i * T);
+}
+
+// Forward core.  ld(family, urow) returns the lane's element of that row; emit(r, k2, which,
+// family, urow, v) receives the unscaled output of that row as soon as it is computed
+// (family FAM_C with the absolute row for the group g = 0).
 template <int n, int THREADS, typename Ld, typename Emit>
-__device__ __forceinline__ void wteam_fwd(int t, int line, Ld ld, float *xl, CConst<n, CTeam<n, THREADS>::M> cc,
+__device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CConst<n, CTeam<n, THREADS>::M> cc,
                                           Emit emit)
 {
     using C = CTeam<n, THREADS>;
-    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
-    const int sl = line % SL, rnd = line / SL;
+    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q;
     float re[N1], im[N1];
 #pragma unroll
     for (int j1 = 0; j1 < N1; j1++) {
-        int i0, i1;
-        team_pack_index<n, T>(t, j1, i0, i1);
-        re[j1] = ld(i0);
-        im[j1] = ld(i1);
+        const int fam = in_family<n, T>(j1), u = in_urow<n, T>(j1);
+        re[j1] = ld(fam, u);
+        im[j1] = ld(fam, fam == FAM_P ? u + 2 : u - 2);
     }
     Dft<N1, -1>::run(re, im);
     __builtin_amdgcn_sched_barrier(0);
@@ -92,153 +192,111 @@ __device__ __forceinline__ void wteam_fwd(int t, int line, Ld ld, float *xl, CCo
         re[k1] = __builtin_fmaf(a, wr, -b * wi);
         im[k1] = __builtin_fmaf(a, wi, b * wr);
     }
-    float dr[Q][2][T], di[Q][2][T];
-#pragma unroll
-    for (int rr = 0; rr < ROUNDS; rr++) {
-        if (rnd == rr) {
-#pragma unroll
-            for (int k1 = 0; k1 < N1; k1++) {
-                const int c = k1 * T + t;
-                xl[(2 * c) * SL + sl] = re[k1];
-                xl[(2 * c + 1) * SL + sl] = im[k1];
-            }
-        }
-        __syncthreads();
-        if (rnd == rr) {
-#pragma unroll
-            for (int r = 0; r < Q; r++) {
-                const int g = t + T * r;
-                const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
-#pragma unroll
-                for (int j2 = 0; j2 < T; j2++) {
-                    dr[r][0][j2] = xl[(2 * (ka * T + j2)) * SL + sl];
-                    di[r][0][j2] = xl[(2 * (ka * T + j2) + 1) * SL + sl];
-                    dr[r][1][j2] = xl[(2 * (kb * T + j2)) * SL + sl];
-                    di[r][1][j2] = xl[(2 * (kb * T + j2) + 1) * SL + sl];
-                }
-            }
-        }
-        __syncthreads();
-    }
+    cteam_exchange<n, THREADS, true>(t, line, re, im, xl);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
-        Dft<T, -1>::run(dr[r][0], di[r][0]);
-        Dft<T, -1>::run(dr[r][1], di[r][1]);
-        if (g == 0) {
+        float (&d0r)[T] = sub<T>(re, 2 * r), (&d0i)[T] = sub<T>(im, 2 * r);
+        float (&d1r)[T] = sub<T>(re, 2 * r + 1), (&d1i)[T] = sub<T>(im, 2 * r + 1);
+        Dft<T, -1>::run(d0r, d0i);
+        Dft<T, -1>::run(d1r, d1i);
+        if (r == 0 && t == 0) {
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 float o[4];
                 if (k2 == T / 2) {
-                    post_fwd_special(dr[r][0][0], di[r][0][0], dr[r][0][T / 2], di[r][0][T / 2],
-                                     cc.tn(n / 4), o);
+                    post_fwd_special(d0r[0], d0i[0], d0r[T / 2], d0i[T / 2], cc.tn(n / 4), o);
                 } else {
                     const bool lo = k2 < T / 2;
                     const int j = k2 - T / 2;
                     const int k = lo ? N1 / 2 + N1 * k2 : N1 * j;
-                    const float ar = lo ? dr[r][1][k2] : dr[r][0][j];
-                    const float ai = lo ? di[r][1][k2] : di[r][0][j];
-                    const float br = lo ? dr[r][1][T - 1 - k2] : dr[r][0][T - j];
-                    const float bi = lo ? di[r][1][T - 1 - k2] : di[r][0][T - j];
+                    const float ar = lo ? d1r[k2] : d0r[j];
+                    const float ai = lo ? d1i[k2] : d0i[j];
+                    const float br = lo ? d1r[T - 1 - k2] : d0r[T - j];
+                    const float bi = lo ? d1i[T - 1 - k2] : d0i[T - j];
                     post_fwd(ar, ai, br, bi, cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), o);
                 }
 #pragma unroll
                 for (int w = 0; w < 4; w++)
-                    emit(r, k2, w, o[w]);
+                    emit(r, k2, w, FAM_C, out_row_g0<n, T>(k2, w), o[w]);
             }
+            // Keeps the compiler from sinking the last stores of both branches into one
+            // block, which would turn their row offsets into a scratch-memory table.
+            __asm__ volatile("" ::: "memory");
         } else {
+            const int g = t + T * r;
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 const int k = g + N1 * k2;
                 float o[4];
-                post_fwd(dr[r][0][k2], di[r][0][k2], dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2],
-                         cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), o);
+                post_fwd(d0r[k2], d0i[k2], d1r[T - 1 - k2], d1i[T - 1 - k2], cc.om_re(k), cc.om_im(k), cc.tn(k),
+                         cc.tn(N - k), o);
 #pragma unroll
                 for (int w = 0; w < 4; w++)
-                    emit(r, k2, w, o[w]);
+                    emit(r, k2, w, out_family<n, T>(w), out_urow<n, T>(r, k2, w), o[w]);
             }
         }
     }
 }
 
-// Inverse core: o[r][k2][w] holds the (already scaled) input of index wteam_index(...);
-// st(i, v) stores element i of the lane's line.
-template <int n, int THREADS, typename St>
-__device__ __forceinline__ void wteam_inv(int t, int line, float (&o)[CTeam<n, THREADS>::Q][CTeam<n, THREADS>::T][4],
-                                          St st, float *xl, CConst<n, CTeam<n, THREADS>::M> cc)
+// Inverse core.  in(r, k2, which, family, urow) returns the (already scaled) input of that
+// output row; st(family, urow, v) stores the lane's element of that row.
+template <int n, int THREADS, typename In, typename St>
+__device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *xl, CConst<n, CTeam<n, THREADS>::M> cc)
 {
     using C = CTeam<n, THREADS>;
-    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
-    const int sl = line % SL, rnd = line / SL;
-    float dr[Q][2][T], di[Q][2][T];
+    constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q;
+    float re[N1], im[N1];
 #pragma unroll
     for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
-        if (g == 0) {
+        float (&d0r)[T] = sub<T>(re, 2 * r), (&d0i)[T] = sub<T>(im, 2 * r);
+        float (&d1r)[T] = sub<T>(re, 2 * r + 1), (&d1i)[T] = sub<T>(im, 2 * r + 1);
+        if (r == 0 && t == 0) {
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
+                float o[4];
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    o[w] = in(r, k2, w, FAM_C, out_row_g0<n, T>(k2, w));
                 if (k2 == T / 2) {
-                    post_inv_special(o[r][k2], cc.tn(n / 4), dr[r][0][0], di[r][0][0], dr[r][0][T / 2],
-                                     di[r][0][T / 2]);
+                    post_inv_special(o, cc.tn(n / 4), d0r[0], d0i[0], d0r[T / 2], d0i[T / 2]);
                     continue;
                 }
                 const bool lo = k2 < T / 2;
                 const int j = k2 - T / 2;
                 const int k = lo ? N1 / 2 + N1 * k2 : N1 * j;
                 float ar, ai, br, bi;
-                post_inv(o[r][k2], cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), ar, ai, br, bi);
+                post_inv(o, cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), ar, ai, br, bi);
                 if (lo) {
-                    dr[r][1][k2] = ar;
-                    di[r][1][k2] = ai;
-                    dr[r][1][T - 1 - k2] = br;
-                    di[r][1][T - 1 - k2] = bi;
+                    d1r[k2] = ar;
+                    d1i[k2] = ai;
+                    d1r[T - 1 - k2] = br;
+                    d1i[T - 1 - k2] = bi;
                 } else {
-                    dr[r][0][j] = ar;
-                    di[r][0][j] = ai;
-                    dr[r][0][T - j] = br;
-                    di[r][0][T - j] = bi;
+                    d0r[j] = ar;
+                    d0i[j] = ai;
+                    d0r[T - j] = br;
+                    d0i[T - j] = bi;
                 }
             }
+            __asm__ volatile("" ::: "memory");   // no sinking across the branches (as above)
         } else {
+            const int g = t + T * r;
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 const int k = g + N1 * k2;
-                post_inv(o[r][k2], cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), dr[r][0][k2],
-                         di[r][0][k2], dr[r][1][T - 1 - k2], di[r][1][T - 1 - k2]);
+                float o[4];
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    o[w] = in(r, k2, w, out_family<n, T>(w), out_urow<n, T>(r, k2, w));
+                post_inv(o, cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), d0r[k2], d0i[k2], d1r[T - 1 - k2],
+                         d1i[T - 1 - k2]);
             }
         }
-        Dft<T, 1>::run(dr[r][0], di[r][0]);
-        Dft<T, 1>::run(dr[r][1], di[r][1]);
+        Dft<T, 1>::run(d0r, d0i);
+        Dft<T, 1>::run(d1r, d1i);
     }
-    float re[N1], im[N1];
-#pragma unroll
-    for (int rr = 0; rr < ROUNDS; rr++) {
-        if (rnd == rr) {
-#pragma unroll
-            for (int r = 0; r < Q; r++) {
-                const int g = t + T * r;
-                const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
-#pragma unroll
-                for (int j2 = 0; j2 < T; j2++) {
-                    xl[(2 * (ka * T + j2)) * SL + sl] = dr[r][0][j2];
-                    xl[(2 * (ka * T + j2) + 1) * SL + sl] = di[r][0][j2];
-                    xl[(2 * (kb * T + j2)) * SL + sl] = dr[r][1][j2];
-                    xl[(2 * (kb * T + j2) + 1) * SL + sl] = di[r][1][j2];
-                }
-            }
-        }
-        __syncthreads();
-        if (rnd == rr) {
-#pragma unroll
-            for (int k1 = 0; k1 < N1; k1++) {
-                const int c = k1 * T + t;
-                re[k1] = xl[(2 * c) * SL + sl];
-                im[k1] = xl[(2 * c + 1) * SL + sl];
-            }
-        }
-        __syncthreads();
-    }
+    cteam_exchange<n, THREADS, false>(t, line, re, im, xl);
 #pragma unroll
     for (int k1 = 1; k1 < N1; k1++) {
         const float wr = cc.tw_re(k1, t), wi = cc.tw_im(k1, t);
@@ -249,10 +307,9 @@ __device__ __forceinline__ void wteam_inv(int t, int line, float (&o)[CTeam<n, T
     Dft<N1, 1>::run(re, im);
 #pragma unroll
     for (int j1 = 0; j1 < N1; j1++) {
-        int i0, i1;
-        team_pack_index<n, T>(t, j1, i0, i1);
-        st(i0, re[j1]);
-        st(i1, im[j1]);
+        const int fam = in_family<n, T>(j1), u = in_urow<n, T>(j1);
+        st(fam, u, re[j1]);
+        st(fam, fam == FAM_P ? u + 2 : u - 2, im[j1]);
     }
 }
 
