@@ -17,19 +17,18 @@
 namespace bdx {
 namespace {
 
-// Lanes per column-pass workgroup: one wave per team member up to the cap (uniform member
+// Lanes per column-pass workgroup: one wave per team member up to 512 lanes (uniform member
 // index: scalar row addresses and constants), else two members per wave.  Measured on
 // 64 x 1024^2 (length-1024 team, 16 members):
 //   forward: 512 lanes (two members per wave, 103 VGPRs, two workgroups per CU) 99.7 us,
 //            1024 lanes (one member per wave, 97 VGPRs, one workgroup per CU) 115.8 us;
-//   inverse: 512 lanes 128.7 us -- the self-paired group g = 0 is a divergent branch in
-//            wave 0 whose results stay in registers, so both branches' values are live at
-//            once (150 VGPRs, one workgroup per CU) -- 1024 lanes 116.0 us (126 VGPRs).
-// The fused round trip keeps 512 lanes: at 1024 its length-1024 team spills.
+//   inverse: 512 lanes (120 VGPRs, two workgroups per CU) 104.5 us; 1024 lanes (one
+//            workgroup per CU) 116.0 us.
+// The fused round trip (141 VGPRs) runs one 512-lane workgroup per CU; at 1024 lanes its
+// length-1024 team spills.
 template <int H, int MODE> constexpr int col_threads()
 {
-    constexpr int cap = MODE == 1 ? 1024 : 512;
-    return H >= 128 ? (64 * team_size<H>() < cap ? 64 * team_size<H>() : cap) : kThreads;
+    return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
 }
 
 template <int W, int OP, bool G, bool NT>

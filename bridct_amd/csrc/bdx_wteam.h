@@ -296,14 +296,27 @@ __device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *
         Dft<T, 1>::run(d0r, d0i);
         Dft<T, 1>::run(d1r, d1i);
     }
-    cteam_exchange<n, THREADS, false>(t, line, re, im, xl);
+    // The twiddles of the DFT_N1 stage are applied in pair order, before the exchange: value
+    // (k1, member j2) of the lane's groups (k1 = ka or kb) times conj(w(k1, j2)), the same
+    // products the member j2 would form after the exchange, k1 = 0 untouched.  After the
+    // exchange the twiddle loads would be scheduled next to the full DFT_N1 working set
+    // (150 VGPRs at two members per wave against 120 here).
 #pragma unroll
-    for (int k1 = 1; k1 < N1; k1++) {
-        const float wr = cc.tw_re(k1, t), wi = cc.tw_im(k1, t);
-        const float a = re[k1], b = im[k1];
-        re[k1] = __builtin_fmaf(a, wr, b * wi);
-        im[k1] = __builtin_fmaf(b, wr, -a * wi);
-    }
+    for (int r = 0; r < Q; r++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int j2 = 0; j2 < T; j2++) {
+                const int g = t + T * r;
+                const int k1 = s == 0 ? g : (g == 0 ? N1 / 2 : N1 - g);
+                const int v = (2 * r + s) * T + j2;
+                const float wr = cc.tw_re(k1, j2), wi = cc.tw_im(k1, j2);
+                const float a = re[v], b = im[v];
+                const float x = __builtin_fmaf(a, wr, b * wi), y = __builtin_fmaf(b, wr, -a * wi);
+                re[v] = k1 == 0 ? a : x;
+                im[v] = k1 == 0 ? b : y;
+            }
+    cteam_exchange<n, THREADS, false>(t, line, re, im, xl);
     Dft<N1, 1>::run(re, im);
 #pragma unroll
     for (int j1 = 0; j1 < N1; j1++) {
