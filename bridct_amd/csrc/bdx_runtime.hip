@@ -49,7 +49,8 @@ __global__ void k_zero(unsigned *p, unsigned n)
 }
 
 // Failure injection for the tests of the allocation-failure paths, and a count of the
-// runtime resources the library holds (device and pinned buffers, streams, events).
+// runtime resources the library holds (device and pinned buffers, streams, events, graph
+// executables).
 // fault_after < 0: off; k >= 0: k more acquisitions succeed, the next one fails, then off.
 std::atomic<long> fault_after{-1};
 std::atomic<long> live_resources{0};
@@ -237,7 +238,7 @@ int bdx_graph_capture(void *stream, int (*fn)(void *), void *arg, void **graph_e
 {
     hipStream_t s = (hipStream_t)stream;
     hipGraph_t graph = nullptr;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    if (injected_fault() || hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return -1;
     const int rc = fn(arg);
     const hipError_t end = hipStreamEndCapture(s, &graph);
@@ -249,7 +250,7 @@ int bdx_graph_capture(void *stream, int (*fn)(void *), void *arg, void **graph_e
     hipGraphExec_t exec = nullptr;
     const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (inst != hipSuccess)
+    if (acquired(inst == hipSuccess))
         return -1;
     *graph_exec = (void *)exec;
     return 0;
@@ -262,7 +263,7 @@ int bdx_graph_launch(void *graph_exec, void *stream)
 
 int bdx_graph_destroy(void *graph_exec)
 {
-    return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? 0 : -1;
+    return released(graph_exec, hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess);
 }
 
 int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t batch, int pack, int mode,

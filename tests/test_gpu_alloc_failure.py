@@ -127,3 +127,32 @@ def test_codec_create_failures():
     q = np.ascontiguousarray(S.JPEG_LUMA, np.float32)
     n = each_failure(lambda: lib.bridct_codec8_create(capi.ptr(q)), lib.bridct_codec8_destroy)
     print('codec: %d acquisitions' % n, flush=True)
+
+
+def test_batch_graph_failure():
+    """The captured graph of bridct_batch_graph_apply is a counted resource: a failed
+    capture reports BRIDCT_DEVICE_ERROR and holds nothing, the next call captures and
+    replays, and destroy gives back every resource."""
+    lib = capi.load()
+    shapes = [(16, 32), (128, 1024), (64, 64), (256, 256)]
+    rng = np.random.default_rng(6)
+    arrays = [rng.uniform(-1, 1, s).astype(np.float32) for s in shapes]
+    x = np.concatenate([a.ravel() for a in arrays])
+    base = lib.bridct_debug_alloc_faults(-1)
+    b = capi.Batch(shapes)
+    d_in, d_out = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    before = lib.bridct_debug_alloc_faults(0)
+    rc = b.graph_apply(capi.FORWARD, d_in.ptr, d_out.ptr)
+    assert lib.bridct_debug_alloc_faults(-1) == before
+    assert rc == capi.DEVICE_ERROR
+    assert b.graph_apply(capi.FORWARD, d_in.ptr, d_out.ptr) == 0 and b.synchronize() == 0
+    assert lib.bridct_debug_alloc_faults(-1) == before + 1
+    y = np.empty_like(x)
+    d_out.download(y)
+    for (h, w), a, out in zip(shapes, arrays, b.split(y)):
+        assert S.max_rel(out, S.oracle_2d(a, 0)) < 1e-6, (h, w)
+    d_in.free()
+    d_out.free()
+    b.close()
+    assert lib.bridct_debug_alloc_faults(-1) == base
