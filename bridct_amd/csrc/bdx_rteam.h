@@ -7,8 +7,9 @@
 // float4s of the mirrored lane T - 1 - t; one DPP mirror (row_mirror for T = 16,
 // row_half_mirror for 8, quad_perm [3,2,1,0] for 4) per value brings them over, with no
 // LDS traffic.  The exchange between the two FFT phases goes through a per-team LDS
-// region, real parts then imaginary parts, with the column index XORed by the row index
-// so that both the writes and the reads of a wave hit distinct banks.  The forward
+// region, real parts then imaginary parts, in columns of T slots padded by one so that
+// both the writes and the reads of a wave hit distinct banks with immediate-offset
+// addresses (an XOR swizzle had cost three to four address instructions per slot).  The forward
 // outputs are scattered over the row, so they are collected in the same region and
 // written back as whole float4 lines (scattered 64-byte stores write partial cache lines
 // and cost extra HBM write traffic).  Per-lane constants (team twiddles, post-process
@@ -33,12 +34,13 @@ template <int T> __device__ __forceinline__ float team_mirror(float x)
 
 // LDS copy of the constants of axis length n used by a team; XCH is the per-team LDS
 // region (forward: FFT exchange, then the output row; inverse: the input row, then the
-// FFT exchange), padded so that the teams of one wave start on different banks.  It holds
-// half a row: the FFT exchange moves n/2 values at a time, and the output (input) row is
-// staged in two rounds of half a row -- outputs 0 and 2 of every slot lie in the first
-// half, 1 and 3 in the second.  At n = 1024 that brings a 16-row workgroup from 80.9 to
-// 48.1 KB of LDS, three workgroups per CU instead of two (2% faster on 64 x 1024^2, equal
-// results bitwise).
+// FFT exchange).  It holds half a row: the FFT exchange moves n/2 values at a time (in
+// N1 columns of T slots padded to T + 1, see xpad), and the output (input) row is staged
+// in two rounds of half a row -- outputs 0 and 2 of every slot lie in the first half, 1
+// and 3 in the second.  At n = 1024 that brings a 16-row workgroup from 80.9 to about 50
+// KB of LDS, three workgroups per CU instead of two.  The region size is rounded up to
+// T modulo 32 floats, so that neighbouring teams start T banks apart and their exchange
+// accesses cover distinct banks (rounding to T modulo 64 measured the same).
 template <int n> struct RowTab {
     static constexpr int N = n / 2;
     static constexpr int TW = 0;                // team twiddles, 2 (k1 T + t)
@@ -46,7 +48,8 @@ template <int n> struct RowTab {
     static constexpr int TN = 2 * n;            // tan(pi k / (2n)), k <= N
     static constexpr int AL = 2 * n + N + 4;    // normalization (alpha or alpha^2)
     static constexpr int SIZE = AL + n;
-    static constexpr int XCH = n / 2 + BDX_TEAM(n);   // half a row: staging runs in two rounds
+    static constexpr int XMIN = n / 2 + n / 2 / BDX_TEAM(n);   // N1 (T + 1) exchange slots >= half a row
+    static constexpr int XCH = XMIN + ((BDX_TEAM(n) - XMIN) % 32 + 32) % 32;
     static_assert(SIZE % 4 == 0 && XCH % 4 == 0, "team regions must stay 16-byte aligned");
 };
 
@@ -64,10 +67,15 @@ __device__ __forceinline__ void load_row_tables(float *lt, const float *__restri
         lt[R::TN + i] = g[2 * n + i];
 }
 
-// Exchange slot of (column k1, member j): k1 T + (j xor (k1 mod T)).
-template <int T> __device__ __forceinline__ int xswz(int k1, int j)
+// Exchange slot of (column k1, member j): k1 (T + 1) + j.  A lane writes one column per
+// member slot (k1 varies at compile time, j = its own member index) and reads the T slots
+// of its pair columns (j varies at compile time, k1 = its pair group), so every address is
+// a per-lane base plus an immediate offset.  Banks: the T lanes of a team write T
+// consecutive slots, and read slots (T + 1) k1 + j with k1 differing by 1 between
+// neighbouring lanes, which spreads them over distinct banks (T + 1 is odd).
+template <int T> __device__ __forceinline__ constexpr int xpad(int k1, int j)
 {
-    return k1 * T + (j ^ (k1 & (T - 1)));
+    return k1 * (T + 1) + j;
 }
 
 template <int n, int OP, bool NT = false>
@@ -101,7 +109,7 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
     float dr[Q][2][T], di[Q][2][T];
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++)
-        xl[xswz<T>(k1, t)] = re[k1];
+        xl[xpad<T>(k1, t)] = re[k1];
     team_sync();
 #pragma unroll
     for (int r = 0; r < Q; r++) {
@@ -109,14 +117,14 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
         const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
 #pragma unroll
         for (int j2 = 0; j2 < T; j2++) {
-            dr[r][0][j2] = xl[xswz<T>(ka, j2)];
-            dr[r][1][j2] = xl[xswz<T>(kb, j2)];
+            dr[r][0][j2] = xl[xpad<T>(ka, j2)];
+            dr[r][1][j2] = xl[xpad<T>(kb, j2)];
         }
     }
     team_sync();
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++)
-        xl[xswz<T>(k1, t)] = im[k1];
+        xl[xpad<T>(k1, t)] = im[k1];
     team_sync();
 #pragma unroll
     for (int r = 0; r < Q; r++) {
@@ -124,8 +132,8 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
         const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
 #pragma unroll
         for (int j2 = 0; j2 < T; j2++) {
-            di[r][0][j2] = xl[xswz<T>(ka, j2)];
-            di[r][1][j2] = xl[xswz<T>(kb, j2)];
+            di[r][0][j2] = xl[xpad<T>(ka, j2)];
+            di[r][1][j2] = xl[xpad<T>(kb, j2)];
         }
     }
     team_sync();   // the region is free for the next line once every lane has read
@@ -324,14 +332,14 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
         const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
 #pragma unroll
         for (int j2 = 0; j2 < T; j2++) {
-            xl[xswz<T>(ka, j2)] = dr[r][0][j2];
-            xl[xswz<T>(kb, j2)] = dr[r][1][j2];
+            xl[xpad<T>(ka, j2)] = dr[r][0][j2];
+            xl[xpad<T>(kb, j2)] = dr[r][1][j2];
         }
     }
     team_sync();
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++)
-        re[k1] = xl[xswz<T>(k1, t)];
+        re[k1] = xl[xpad<T>(k1, t)];
     team_sync();
 #pragma unroll
     for (int r = 0; r < Q; r++) {
@@ -339,14 +347,14 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
         const int ka = g == 0 ? 0 : g, kb = g == 0 ? N1 / 2 : N1 - g;
 #pragma unroll
         for (int j2 = 0; j2 < T; j2++) {
-            xl[xswz<T>(ka, j2)] = di[r][0][j2];
-            xl[xswz<T>(kb, j2)] = di[r][1][j2];
+            xl[xpad<T>(ka, j2)] = di[r][0][j2];
+            xl[xpad<T>(kb, j2)] = di[r][1][j2];
         }
     }
     team_sync();
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++)
-        im[k1] = xl[xswz<T>(k1, t)];
+        im[k1] = xl[xpad<T>(k1, t)];
     team_sync();
 #pragma unroll
     for (int k1 = 1; k1 < N1; k1++) {
