@@ -31,9 +31,14 @@ template <int H, int MODE> constexpr int col_threads()
     return H >= 128 ? (64 * team_size<H>() < 512 ? 64 * team_size<H>() : 512) : kThreads;
 }
 
+// Rows of 512 and 1024 fit three workgroups per CU by LDS; the bound keeps their
+// registers within three waves per SIMD (168 VGPRs).
+template <int W> constexpr int rows_min_blocks() { return W >= 512 ? 3 : 1; }
+
 template <int W, int OP, bool G, bool NT>
-__global__ __launch_bounds__(kThreads) void k_rows(const float *in, float *out, const size_t *__restrict__ offs,
-                                                    int h, const float *__restrict__ tab)
+__global__ __launch_bounds__(kThreads, rows_min_blocks<W>()) void k_rows(const float *in, float *out,
+                                                                          const size_t *__restrict__ offs, int h,
+                                                                          const float *__restrict__ tab)
 {
     __shared__ float smem[row_unit_lds<W, OP, kThreads>()];
     constexpr int R = row_unit_rows<W, kThreads>();

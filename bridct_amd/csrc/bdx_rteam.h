@@ -78,6 +78,46 @@ template <int T> __device__ __forceinline__ constexpr int xpad(int k1, int j)
     return k1 * (T + 1) + j;
 }
 
+// Per-lane pointers of the post-process slots of a team.  Slot k2 of pair group
+// g = base + T r (r < Q) handles the pair (k, N - k), k = base + T r + N1 k2.  Each pointer
+// sits at the lowest address its family reaches, so every access is a pointer plus a
+// non-negative immediate offset:
+//   om[2k], om[2k+1]           om + 2 u           (u = T r + N1 k2)
+//   tan(k), al[k], al[N + k]    up + {0, AL - TN, AL - TN + N} + u
+//   tan(N - k), al[N - k],      dn + {0, AL - TN, AL - TN + N} + d   (d = SPAN - u)
+//   al[n - k]
+//   half-row slots [k], [N - k] xu + u, xd + d
+template <int n> struct SlotPtrs {
+    using R = RowTab<n>;
+    static constexpr int T = BDX_TEAM(n), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
+    static constexpr int SPAN = T * (Q - 1) + N1 * (T - 1);   // largest u
+    static constexpr int A = R::AL - R::TN;
+    static_assert(N - T - SPAN >= 0, "the downward pointers stay inside their regions");
+    const float *om, *up, *dn;
+    float *xu, *xd;
+    __device__ __forceinline__ SlotPtrs(const float *lt, float *xl, int base)
+        : om(lt + R::OM + 2 * base), up(lt + R::TN + base), dn(lt + R::TN + N - base - SPAN), xu(xl + base),
+          xd(xl + N - base - SPAN)
+    {
+    }
+    static constexpr int u(int r, int k2) { return T * r + N1 * k2; }
+    static constexpr int d(int r, int k2) { return SPAN - u(r, k2); }
+    __device__ __forceinline__ float om_re(int r, int k2) const { return om[2 * u(r, k2)]; }
+    __device__ __forceinline__ float om_im(int r, int k2) const { return om[2 * u(r, k2) + 1]; }
+    __device__ __forceinline__ float tn_k(int r, int k2) const { return up[u(r, k2)]; }
+    __device__ __forceinline__ float tn_nk(int r, int k2) const { return dn[d(r, k2)]; }
+    // normalization of output w of the slot: out[k], out[n - k], out[N - k], out[N + k]
+    __device__ __forceinline__ float al(int r, int k2, int w) const
+    {
+        return w == 0 ? up[A + u(r, k2)] : w == 1 ? dn[A + N + d(r, k2)] : w == 2 ? dn[A + d(r, k2)] : up[A + N + u(r, k2)];
+    }
+    // half-row slot of output w (outputs 1 and 3 in the second half, local index - N)
+    __device__ __forceinline__ float &x(int r, int k2, int w) const
+    {
+        return w == 0 || w == 3 ? xu[u(r, k2)] : xd[d(r, k2)];
+    }
+};
+
 template <int n, int OP, bool NT = false>
 __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, float *xl, const float *lt)
 {
@@ -138,63 +178,64 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
     }
     team_sync();   // the region is free for the next line once every lane has read
     __builtin_amdgcn_sched_barrier(0);
+    // Slot k2 of pair group g = t + T r yields the outputs k, n - k, N - k and N + k of the
+    // pair (k, N - k), k = g + N1 k2, from (D0[k2], D1[T-1-k2]).  Lane t = 0 of group r = 0
+    // owns the self-paired groups 0 and N1/2 instead (RowSlots): its low slots take k = N1/2 +
+    // N1 k2 from (D1[k2], D1[T-1-k2]), its high slots k = N1 k2 from (D0[k2], D0[T-k2]) --
+    // the pair (N - k, k) seen from its other side, which post_fwd maps to the same four
+    // outputs bitwise -- and its slot T/2 holds Z_0 and Z_{N/2}.  Every slot's k is thus a
+    // per-lane base plus N1 k2, and every table and output address a per-lane base plus an
+    // immediate offset.
+    const bool lead = t == 0;
+    const SlotPtrs<n> lo(lt, xl, lead ? N1 / 2 : t), hi(lt, xl, t);   // slots k2 < T/2 of group 0; the others
     float second[Q][T][2];
+    float lead_first = 0.f, lead_second = 0.f;   // out[0] and out[N] (lane t = 0 only)
 #pragma unroll
     for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
-        const bool g0 = g == 0;
         Dft<T, -1>::run(dr[r][0], di[r][0]);
         Dft<T, -1>::run(dr[r][1], di[r][1]);
 #pragma unroll
         for (int k2 = 0; k2 < T; k2++) {
-            float ar, ai, br, bi;
-            int k;
-            if (k2 < T / 2) {
-                ar = g0 ? dr[r][1][k2] : dr[r][0][k2];
-                ai = g0 ? di[r][1][k2] : di[r][0][k2];
-                br = dr[r][1][T - 1 - k2];
-                bi = di[r][1][T - 1 - k2];
-                k = g0 ? N1 / 2 + N1 * k2 : g + N1 * k2;
-            } else if (k2 > T / 2) {
-                const int j = k2 - T / 2;
-                ar = g0 ? dr[r][0][j] : dr[r][0][k2];
-                ai = g0 ? di[r][0][j] : di[r][0][k2];
-                br = g0 ? dr[r][0][T - j] : dr[r][1][T - 1 - k2];
-                bi = g0 ? di[r][0][T - j] : di[r][1][T - 1 - k2];
-                k = g0 ? N1 * j : g + N1 * k2;
-            } else {
-                ar = dr[r][0][k2];
-                ai = di[r][0][k2];
-                br = dr[r][1][T - 1 - k2];
-                bi = di[r][1][T - 1 - k2];
-                k = g0 ? 1 : g + N1 * k2;
-            }
+            const SlotPtrs<n> &P = r == 0 && k2 < T / 2 ? lo : hi;
+            const bool swap_a = r == 0 && k2 < T / 2, swap_b = r == 0 && k2 > T / 2;
+            const float ar = swap_a && lead ? dr[r][1][k2] : dr[r][0][k2];
+            const float ai = swap_a && lead ? di[r][1][k2] : di[r][0][k2];
+            const float br = swap_b && lead ? dr[r][0][T - k2] : dr[r][1][T - 1 - k2];
+            const float bi = swap_b && lead ? di[r][0][T - k2] : di[r][1][T - 1 - k2];
             float o[4];
-            post_fwd(ar, ai, br, bi, lt[R::OM + 2 * k], lt[R::OM + 2 * k + 1], lt[R::TN + k],
-                     lt[R::TN + N - k], o);
-            if (k2 == T / 2) {
-                float s[4];
-                post_fwd_special(dr[r][0][0], di[r][0][0], dr[r][0][T / 2], di[r][0][T / 2],
-                                 lt[R::TN + n / 4], s);
+            post_fwd(ar, ai, br, bi, P.om_re(r, k2), P.om_im(r, k2), P.tn_k(r, k2), P.tn_nk(r, k2), o);
+            if (r == 0 && k2 == T / 2) {
+                // lane 0: k = N/2, so its four outputs land on n/4, 3n/4, n/4, 3n/4; it
+                // writes out[n/4] and out[3n/4] there (twice each) and out[0], out[N] apart
+                float sp[4];
+                post_fwd_special(dr[0][0][0], di[0][0][0], dr[0][0][T / 2], di[0][0][T / 2],
+                                 lt[R::TN + n / 4], sp);
 #pragma unroll
                 for (int w = 0; w < 4; w++)
-                    o[w] = g0 ? s[w] : o[w];
+                    o[w] = lead ? sp[2 + (w & 1)] : o[w];
+                lead_first = sp[0];
+                lead_second = sp[1];
+                if constexpr (OP == FA || OP == FA2) {
+                    lead_first *= lt[R::AL];
+                    lead_second *= lt[R::AL + N];
+                }
             }
             // outputs 0 and 2 of a slot lie in the first half of the row, 1 and 3 in the
             // second: the first go to the half-row region now, the second wait in registers
 #pragma unroll
             for (int w = 0; w < 4; w++) {
-                const int idx = team_out_index<n, T>(g0, g, k2, w);
                 float y = o[w];
                 if constexpr (OP == FA || OP == FA2)
-                    y *= lt[R::AL + idx];
+                    y *= P.al(r, k2, w);
                 if (w % 2 == 0)
-                    xl[idx] = y;
+                    P.x(r, k2, w) = y;
                 else
                     second[r][k2][w / 2] = y;
             }
         }
     }
+    if (lead)
+        xl[0] = lead_first;
     team_sync();
 #pragma unroll
     for (int c = 0; c < N / (4 * T); c++) {
@@ -203,14 +244,15 @@ __device__ __forceinline__ void rteam_fwd(const float *src, float *dst, int t, f
     }
     team_sync();
 #pragma unroll
-    for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
+    for (int r = 0; r < Q; r++)
 #pragma unroll
-        for (int k2 = 0; k2 < T; k2++)
-#pragma unroll
-            for (int w = 1; w < 4; w += 2)
-                xl[team_out_index<n, T>(g == 0, g, k2, w) - N] = second[r][k2][w / 2];
-    }
+        for (int k2 = 0; k2 < T; k2++) {
+            const SlotPtrs<n> &P = r == 0 && k2 < T / 2 ? lo : hi;
+            P.x(r, k2, 1) = second[r][k2][0];
+            P.x(r, k2, 3) = second[r][k2][1];
+        }
+    if (lead)
+        xl[0] = lead_second;
     team_sync();
 #pragma unroll
     for (int c = 0; c < N / (4 * T); c++) {
@@ -240,85 +282,73 @@ __device__ __forceinline__ void rteam_inv(const float *src, float *dst, int t, f
     for (int c = 0; c < N / (4 * T); c++)
         *reinterpret_cast<float4 *>(xl + 4 * t + 4 * T * c) = v[c];
     team_sync();
+    // Slots as in rteam_fwd: slot k2 of pair group g = t + T r reads the inputs at k, n - k,
+    // N - k and N + k, k = per-lane base + N1 k2; lane t = 0 of group 0 takes the base N1/2
+    // below T/2, and at slot T/2 (k = N/2) finds in[n/4] and in[3n/4], with in[0] and in[N]
+    // read apart.
+    const bool lead = t == 0;
+    const SlotPtrs<n> lo(lt, xl, lead ? N1 / 2 : t), hi(lt, xl, t);
+    const auto scaled = [&](float y, float a) {
+        if constexpr (OP == IA)
+            y *= a;
+        return y;
+    };
     float first[Q][T][2];
 #pragma unroll
-    for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
+    for (int r = 0; r < Q; r++)
 #pragma unroll
-        for (int k2 = 0; k2 < T; k2++)
-#pragma unroll
-            for (int w = 0; w < 4; w += 2) {
-                const int idx = team_out_index<n, T>(g == 0, g, k2, w);
-                float y = xl[idx];
-                if constexpr (OP == IA)
-                    y *= lt[R::AL + idx];
-                first[r][k2][w / 2] = y;
-            }
-    }
+        for (int k2 = 0; k2 < T; k2++) {
+            const SlotPtrs<n> &P = r == 0 && k2 < T / 2 ? lo : hi;
+            first[r][k2][0] = scaled(P.x(r, k2, 0), P.al(r, k2, 0));
+            first[r][k2][1] = scaled(P.x(r, k2, 2), P.al(r, k2, 2));
+        }
+    const float lead_in0 = scaled(xl[0], lt[R::AL]);   // in[0] (used by lane 0 only)
     team_sync();
 #pragma unroll
     for (int c = 0; c < N / (4 * T); c++)
         *reinterpret_cast<float4 *>(xl + 4 * t + 4 * T * c) = v[N / (4 * T) + c];
     team_sync();
+    const float lead_inN = scaled(xl[0], lt[R::AL + N]);   // in[N]
     float dr[Q][2][T], di[Q][2][T];
 #pragma unroll
     for (int r = 0; r < Q; r++) {
-        const int g = t + T * r;
-        const bool g0 = g == 0;
-        float zar[T], zai[T], zbr[T], zbi[T];
+        // A, B: Z_k and Z_{N-k} of slot k2
+        float ar[T], ai[T], br[T], bi[T];
         float z0r = 0.f, z0i = 0.f, zhr = 0.f, zhi = 0.f;
 #pragma unroll
         for (int k2 = 0; k2 < T; k2++) {
-            float o[4];
-#pragma unroll
-            for (int w = 0; w < 4; w++) {
-                if (w % 2 == 0) {
-                    o[w] = first[r][k2][w / 2];
-                    continue;
-                }
-                const int idx = team_out_index<n, T>(g0, g, k2, w);
-                float y = xl[idx - N];
-                if constexpr (OP == IA)
-                    y *= lt[R::AL + idx];
-                o[w] = y;
+            const SlotPtrs<n> &P = r == 0 && k2 < T / 2 ? lo : hi;
+            const float o[4] = {first[r][k2][0], scaled(P.x(r, k2, 1), P.al(r, k2, 1)), first[r][k2][1],
+                                scaled(P.x(r, k2, 3), P.al(r, k2, 3))};
+            post_inv(o, P.om_re(r, k2), P.om_im(r, k2), P.tn_k(r, k2), P.tn_nk(r, k2), ar[k2], ai[k2], br[k2],
+                     bi[k2]);
+            if (r == 0 && k2 == T / 2) {
+                const float os[4] = {lead_in0, lead_inN, o[0], o[1]};
+                post_inv_special(os, lt[R::TN + n / 4], z0r, z0i, zhr, zhi);
             }
-            int k;
-            if (k2 < T / 2)
-                k = g0 ? N1 / 2 + N1 * k2 : g + N1 * k2;
-            else if (k2 > T / 2)
-                k = g0 ? N1 * (k2 - T / 2) : g + N1 * k2;
-            else
-                k = g0 ? 1 : g + N1 * k2;
-            post_inv(o, lt[R::OM + 2 * k], lt[R::OM + 2 * k + 1], lt[R::TN + k], lt[R::TN + N - k],
-                     zar[k2], zai[k2], zbr[k2], zbi[k2]);
-            if (k2 == T / 2)
-                post_inv_special(o, lt[R::TN + n / 4], z0r, z0i, zhr, zhi);
         }
+        // D0[x] = Z_{g + N1 x}, D1[x] = Z_{N1 - g + N1 x}: from slot x and from the b side of
+        // slot T-1-x; lane 0 of group 0 (groups 0 and N1/2) takes D0[0] = Z_0, D0[T/2] =
+        // Z_{N/2}, D0[x] from the b side of its high slot T - x (x < T/2), D1[x] from the a
+        // side of its low slot x (x < T/2).
+        const bool l0 = r == 0 && lead;
 #pragma unroll
         for (int x = 0; x < T; x++) {
-            float gr, gi;   // lane g = 0 placement of D0[x]
+            float pr = ar[x], pi = ai[x];
             if (x == 0) {
-                gr = z0r;
-                gi = z0i;
+                pr = l0 ? z0r : pr;
+                pi = l0 ? z0i : pi;
             } else if (x == T / 2) {
-                gr = zhr;
-                gi = zhi;
+                pr = l0 ? zhr : pr;
+                pi = l0 ? zhi : pi;
             } else if (x < T / 2) {
-                gr = zar[T / 2 + x];
-                gi = zai[T / 2 + x];
-            } else {
-                gr = zbr[T / 2 + T - x];
-                gi = zbi[T / 2 + T - x];
+                pr = l0 ? br[T - x] : pr;
+                pi = l0 ? bi[T - x] : pi;
             }
-            dr[r][0][x] = g0 ? gr : zar[x];
-            di[r][0][x] = g0 ? gi : zai[x];
-            if (x < T / 2) {
-                dr[r][1][x] = g0 ? zar[x] : zbr[T - 1 - x];
-                di[r][1][x] = g0 ? zai[x] : zbi[T - 1 - x];
-            } else {
-                dr[r][1][x] = zbr[T - 1 - x];
-                di[r][1][x] = zbi[T - 1 - x];
-            }
+            dr[r][0][x] = pr;
+            di[r][0][x] = pi;
+            dr[r][1][x] = x < T / 2 && l0 ? ar[x] : br[T - 1 - x];
+            di[r][1][x] = x < T / 2 && l0 ? ai[x] : bi[T - 1 - x];
         }
         Dft<T, 1>::run(dr[r][0], di[r][0]);
         Dft<T, 1>::run(dr[r][1], di[r][1]);

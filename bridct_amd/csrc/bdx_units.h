@@ -157,9 +157,6 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         // offsets fit 32 bits).
         const unsigned w4 = 4u * (unsigned)w, l4 = 4u * (unsigned)line;
         const int row_p = 4 * t, row_r = 4 * (T - 1 - t), row_c = t, row_d = T - 1 - t;
-        const auto lane_row = [=](int fam) {
-            return fam == FAM_P ? row_p : fam == FAM_R ? row_r : fam == FAM_C ? row_c : row_d;
-        };
         const unsigned off_p = (unsigned)row_p * w4 + l4, off_r = (unsigned)row_r * w4 + l4;
         const unsigned off_c = (unsigned)row_c * w4 + l4, off_d = (unsigned)row_d * w4 + l4;
         const auto off = [=](int fam) {
@@ -175,8 +172,12 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
                 st1<NT>(reinterpret_cast<float *>(dst + (size_t)urow * w4 + off(fam)), v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
-        const auto alpha = [&](int fam, int urow) { return al[lane_row(fam) + urow]; };
-        const CConst<H, M> cc{tab + BDX_TABLE_BASE(H), lt};
+        // one pointer per row family, so that each factor is that pointer plus an immediate
+        const float *al_p = al + row_p, *al_r = al + row_r, *al_c = al + row_c, *al_d = al + row_d;
+        const auto alpha = [&](int fam, int urow) {
+            return (fam == FAM_P ? al_p : fam == FAM_R ? al_r : fam == FAM_C ? al_c : al_d)[urow];
+        };
+        const CConst<H, M> cc(tab + BDX_TABLE_BASE(H), lt, t);
         if constexpr (MODE == 1) {
             cteam_inv<H, THREADS>(
                 t, line, [&](int, int, int, int fam, int urow) { return ld(fam, urow) * alpha(fam, urow); }, st, xl,

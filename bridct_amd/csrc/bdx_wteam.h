@@ -47,21 +47,70 @@ template <int n, int THREADS> struct CTeam {
 };
 
 // Constants of member t: from the global table block (M = 1, wave-uniform) or from the
-// LDS copy (M > 1).
+// LDS copy (M > 1).  With M > 1 the member index varies across the wave, so the lane keeps
+// one pointer per family of constants, each at the lowest address the family reaches, and
+// every access is that pointer plus an immediate offset (as SlotPtrs in bdx_rteam.h; the
+// compiler had turned g + N1 k2 into g | N1 k2, which does not fold into an offset).
+//   twiddle (k1, t)                          tw_lane + 2 T k1
+//   slot k = t + T r + N1 k2: om[2k], tan(k)  om + 2 u, up + u      (u = T r + N1 k2)
+//   tan(N - k)                               dn + SPAN - u
+//   inverse pair twiddles (k1, j2):          twa + 2 T T r + 2 j2                 (k1 = g)
+//                                            twb0 + 2 j2 (r = 0), twb + 2 T T (Q-1-r) + 2 j2
+//                                                                  (k1 = N1 - g, N1/2 at g = 0)
 template <int n, int M> struct CConst {
-    static constexpr int T = team_size<n>();
+    using R = RowTab<n>;
+    static constexpr int T = team_size<n>(), N = n / 2, N1 = N / T, Q = N1 / (2 * T);
+    static constexpr int SPAN = T * (Q - 1) + N1 * (T - 1);   // largest u
     const float *g, *l;
-    __device__ __forceinline__ float tw_re(int k1, int t) const
+    int t;
+    const float *tw_lane, *om, *up, *dn, *twa, *twb0, *twb;
+    __device__ __forceinline__ CConst(const float *g_, const float *l_, int t_)
+        : g(g_), l(l_), t(t_), tw_lane(l_ + R::TW + 2 * t_), om(l_ + R::OM + 2 * t_), up(l_ + R::TN + t_),
+          dn(l_ + R::TN + N - t_ - SPAN), twa(l_ + R::TW + 2 * T * t_),
+          twb0(l_ + R::TW + 2 * T * (t_ == 0 ? N1 / 2 : N1 - t_)), twb(l_ + R::TW + 2 * T * (N1 - t_ - T * (Q - 1)))
     {
-        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1))] : l[RowTab<n>::TW + 2 * (k1 * T + t)];
     }
-    __device__ __forceinline__ float tw_im(int k1, int t) const
+    static constexpr int u(int r, int k2) { return T * r + N1 * k2; }
+    // twiddle (k1, t) of the lane's own member t
+    __device__ __forceinline__ float twl_re(int k1) const
     {
-        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1)) + 1] : l[RowTab<n>::TW + 2 * (k1 * T + t) + 1];
+        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1))] : tw_lane[2 * T * k1];
     }
-    __device__ __forceinline__ float om_re(int k) const { return M == 1 ? g[2 * k] : l[RowTab<n>::OM + 2 * k]; }
-    __device__ __forceinline__ float om_im(int k) const { return M == 1 ? g[2 * k + 1] : l[RowTab<n>::OM + 2 * k + 1]; }
-    __device__ __forceinline__ float tn(int k) const { return M == 1 ? g[2 * n + k] : l[RowTab<n>::TN + k]; }
+    __device__ __forceinline__ float twl_im(int k1) const
+    {
+        return M == 1 ? g[2 * ((2 * t * k1) & (n - 1)) + 1] : tw_lane[2 * T * k1 + 1];
+    }
+    // constants of slot k = t + T r + N1 k2
+    __device__ __forceinline__ float oms_re(int r, int k2) const
+    {
+        return M == 1 ? g[2 * (t + u(r, k2))] : om[2 * u(r, k2)];
+    }
+    __device__ __forceinline__ float oms_im(int r, int k2) const
+    {
+        return M == 1 ? g[2 * (t + u(r, k2)) + 1] : om[2 * u(r, k2) + 1];
+    }
+    __device__ __forceinline__ float tns(int r, int k2) const { return M == 1 ? g[2 * n + t + u(r, k2)] : up[u(r, k2)]; }
+    __device__ __forceinline__ float tnns(int r, int k2) const
+    {
+        return M == 1 ? g[2 * n + N - t - u(r, k2)] : dn[SPAN - u(r, k2)];
+    }
+    // twiddle (k1, j2) of the inverse's pair group s of group r: k1 = g (s = 0) or N1 - g,
+    // N1/2 for g = 0 (s = 1), g = t + T r
+    __device__ __forceinline__ float twp(int r, int s, int j2, int im) const
+    {
+        if constexpr (M == 1) {
+            const int gg = t + T * r, k1 = s == 0 ? gg : (gg == 0 ? N1 / 2 : N1 - gg);
+            return g[2 * ((2 * j2 * k1) & (n - 1)) + im];
+        } else {
+            return s == 0 ? twa[2 * T * T * r + 2 * j2 + im]
+                 : r == 0 ? twb0[2 * j2 + im]
+                          : twb[2 * T * T * (Q - 1 - r) + 2 * j2 + im];
+        }
+    }
+    // absolute index k (compile-time: the group g = 0 of member 0)
+    __device__ __forceinline__ float om_re(int k) const { return M == 1 ? g[2 * k] : l[R::OM + 2 * k]; }
+    __device__ __forceinline__ float om_im(int k) const { return M == 1 ? g[2 * k + 1] : l[R::OM + 2 * k + 1]; }
+    __device__ __forceinline__ float tn(int k) const { return M == 1 ? g[2 * n + k] : l[R::TN + k]; }
 };
 
 // Row families (lane parts): 0 = P (4t), 1 = R (4(T-1-t)), 2 = C (t), 3 = D (T-1-t).
@@ -110,12 +159,18 @@ __device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTea
 {
     using C = CTeam<n, THREADS>;
     constexpr int T = C::T, N1 = C::N1, Q = C::Q, SL = C::SL, ROUNDS = C::ROUNDS;
+    // Real parts in the first half of the area, imaginary parts in the second (at IM), slot
+    // (k1, j) at (k1 T + j) SL + sl: a lane's writes of k1 and k1 + 1 are 64 floats apart times
+    // a whole number, and its reads of j2 and j2 + 1 SL floats apart, so every access is one
+    // base register plus an immediate offset (interleaved re/im pairs had needed an address
+    // add per slot).
+    constexpr int IM = N1 * T * SL;
     const int sl = line % SL, rnd = line / SL;
-    const auto lane_slot = [&](int k1) { return 2 * (k1 * T + t) * SL + sl; };
+    const auto lane_slot = [&](int k1) { return (k1 * T + t) * SL + sl; };
     const auto pair_slot = [&](int r, int s, int j2) {
         const int g = t + T * r;
         const int k = s == 0 ? (g == 0 ? 0 : g) : (g == 0 ? N1 / 2 : N1 - g);
-        return 2 * (k * T + j2) * SL + sl;
+        return (k * T + j2) * SL + sl;
     };
 #pragma unroll
     for (int rr = 0; rr < ROUNDS; rr++) {
@@ -124,7 +179,7 @@ __device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTea
 #pragma unroll
                 for (int k1 = 0; k1 < N1; k1++) {
                     xl[lane_slot(k1)] = re[k1];
-                    xl[lane_slot(k1) + SL] = im[k1];
+                    xl[IM + lane_slot(k1)] = im[k1];
                 }
             } else {
 #pragma unroll
@@ -134,7 +189,7 @@ __device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTea
 #pragma unroll
                         for (int j2 = 0; j2 < T; j2++) {
                             xl[pair_slot(r, s, j2)] = re[(2 * r + s) * T + j2];
-                            xl[pair_slot(r, s, j2) + SL] = im[(2 * r + s) * T + j2];
+                            xl[IM + pair_slot(r, s, j2)] = im[(2 * r + s) * T + j2];
                         }
             }
         }
@@ -148,13 +203,13 @@ __device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTea
 #pragma unroll
                         for (int j2 = 0; j2 < T; j2++) {
                             re[(2 * r + s) * T + j2] = xl[pair_slot(r, s, j2)];
-                            im[(2 * r + s) * T + j2] = xl[pair_slot(r, s, j2) + SL];
+                            im[(2 * r + s) * T + j2] = xl[IM + pair_slot(r, s, j2)];
                         }
             } else {
 #pragma unroll
                 for (int k1 = 0; k1 < N1; k1++) {
                     re[k1] = xl[lane_slot(k1)];
-                    im[k1] = xl[lane_slot(k1) + SL];
+                    im[k1] = xl[IM + lane_slot(k1)];
                 }
             }
         }
@@ -187,7 +242,7 @@ __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CCo
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k1 = 1; k1 < N1; k1++) {
-        const float wr = cc.tw_re(k1, t), wi = cc.tw_im(k1, t);
+        const float wr = cc.twl_re(k1), wi = cc.twl_im(k1);
         const float a = re[k1], b = im[k1];
         re[k1] = __builtin_fmaf(a, wr, -b * wi);
         im[k1] = __builtin_fmaf(a, wi, b * wr);
@@ -224,13 +279,11 @@ __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CCo
             // block, which would turn their row offsets into a scratch-memory table.
             __asm__ volatile("" ::: "memory");
         } else {
-            const int g = t + T * r;
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
-                const int k = g + N1 * k2;
                 float o[4];
-                post_fwd(d0r[k2], d0i[k2], d1r[T - 1 - k2], d1i[T - 1 - k2], cc.om_re(k), cc.om_im(k), cc.tn(k),
-                         cc.tn(N - k), o);
+                post_fwd(d0r[k2], d0i[k2], d1r[T - 1 - k2], d1i[T - 1 - k2], cc.oms_re(r, k2), cc.oms_im(r, k2),
+                         cc.tns(r, k2), cc.tnns(r, k2), o);
 #pragma unroll
                 for (int w = 0; w < 4; w++)
                     emit(r, k2, w, out_family<n, T>(w), out_urow<n, T>(r, k2, w), o[w]);
@@ -281,16 +334,14 @@ __device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *
             }
             __asm__ volatile("" ::: "memory");   // no sinking across the branches (as above)
         } else {
-            const int g = t + T * r;
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
-                const int k = g + N1 * k2;
                 float o[4];
 #pragma unroll
                 for (int w = 0; w < 4; w++)
                     o[w] = in(r, k2, w, out_family<n, T>(w), out_urow<n, T>(r, k2, w));
-                post_inv(o, cc.om_re(k), cc.om_im(k), cc.tn(k), cc.tn(N - k), d0r[k2], d0i[k2], d1r[T - 1 - k2],
-                         d1i[T - 1 - k2]);
+                post_inv(o, cc.oms_re(r, k2), cc.oms_im(r, k2), cc.tns(r, k2), cc.tnns(r, k2), d0r[k2], d0i[k2],
+                         d1r[T - 1 - k2], d1i[T - 1 - k2]);
             }
         }
         Dft<T, 1>::run(d0r, d0i);
@@ -310,7 +361,7 @@ __device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *
                 const int g = t + T * r;
                 const int k1 = s == 0 ? g : (g == 0 ? N1 / 2 : N1 - g);
                 const int v = (2 * r + s) * T + j2;
-                const float wr = cc.tw_re(k1, j2), wi = cc.tw_im(k1, j2);
+                const float wr = cc.twp(r, s, j2, 0), wi = cc.twp(r, s, j2, 1);
                 const float a = re[v], b = im[v];
                 const float x = __builtin_fmaf(a, wr, b * wi), y = __builtin_fmaf(b, wr, -a * wi);
                 re[v] = k1 == 0 ? a : x;
