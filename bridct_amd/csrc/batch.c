@@ -10,13 +10,14 @@
  *   - larger arrays: ROW items (bands of rows, input -> output), COL items (strips of
  *     columns, in place, each waiting for all ROW items of its array) and, for the fused
  *     round trip, ROW2 items (inverse rows, waiting for all COL items).
- * Queue order: two-pass arrays by decreasing size; the COL items of an array are queued
- * `lag` items after its last ROW item (and ROW2 after COL likewise), so that the column
- * strips of one array run while the rows of later arrays are in flight.  With a lag of
- * three times the resident workgroups, the rows an item depends on have almost always
- * completed when it is taken (measured waits of a few microseconds instead of tens); SMALL items go between the last ROW items and the final COL/ROW2 items.  Every
- * item only waits for items earlier in the queue, which were taken by workgroups that are
- * running, so the schedule cannot deadlock.
+ * Queues: the work is cut into units, each two-pass array (by decreasing size) with all its
+ * items, then each SMALL item.  The persistent kernel keeps one queue per XCD: the
+ * workgroups of an XCD draw tickets from it, and slot j of the queue is the ROW items of the
+ * unit the queue takes j-th (from a global unit counter), then the COL items of its
+ * previous unit and, for the round trip, the ROW2 items of the one before.  All items of an
+ * array thus run on one XCD, whose L2 holds the row pass's output for the column pass, and
+ * a dependency only waits for items of earlier tickets of the same queue, taken by running
+ * workgroups: the schedule cannot deadlock.
  *
  * A second strategy, GROUPED, runs the arrays of each distinct shape with the plan path's
  * own per-shape kernels in gather form (the arrays of a shape are not contiguous in the
@@ -106,6 +107,8 @@ struct bridct_batch {
     bdx_run *d_runs;
     bdx_item *d_items[2];    /* [0] forward / inverse order, [1] fused round trip */
     unsigned nitems[2];
+    bdx_unit *d_units;       /* the persistent kernel's units */
+    unsigned nunits, qbase;
     unsigned *d_ctr;
     unsigned nctr;
     float *d_in, *d_out;     /* staging for host-pointer calls, allocated on first use */
@@ -164,58 +167,62 @@ static int push_units(item_list *l, const two_pass *a, unsigned c, unsigned kind
     return 0;
 }
 
-/* Builds the queue of one mode family (fused = round trip with ROW2 items). */
-static int build_items(item_list *l, const bdx_run *runs, unsigned nruns, const two_pass *tp,
-                       unsigned ntwo, int fused, unsigned lag, int tiny)
+/* Builds the units and the item lists of both mode families ([0] forward / inverse, [1] round
+ * trip): each two-pass array (tp, by decreasing size) is a unit of its ROW items, its COL
+ * items and, in the round-trip list, its ROW2 items; then each SMALL item (up to
+ * BDX_BATCH_PER whole arrays of one run, BDX_TINY_PER for tiny batches) is a unit. */
+static int build_units(bdx_schedule *s, const two_pass *tp, item_list lists[2])
 {
-    /* end position (queue index after the last item) of each array's previous stage */
-    unsigned *row_end = malloc((ntwo + 1) * sizeof(unsigned));
-    unsigned *col_end = malloc((ntwo + 1) * sizeof(unsigned));
-    if (!row_end || !col_end) {
-        free(row_end);
-        free(col_end);
+    unsigned nsmall = 0;
+    for (unsigned r = 0; r < s->nruns; r++) {
+        const int h = (int)s->runs[r].h, w = (int)s->runs[r].w;
+        if (BDX_BATCH_SMALL(h, w)) {
+            const unsigned per = (unsigned)(s->tiny ? BDX_TINY_PER(h, w) : BDX_BATCH_PER(h, w));
+            nsmall += (s->runs[r].count + per - 1) / per;
+        }
+    }
+    s->units = malloc((size_t)(s->ntwo + nsmall) * sizeof(bdx_unit));
+    if (!s->units)
         return -1;
-    }
-    unsigned next_col = 0, next_row2 = 0;
     int rc = 0;
-    for (unsigned k = 0; k < ntwo && !rc; k++) {
-        while (!rc && next_col < k && l->n >= row_end[next_col] + lag) {
-            rc = push_units(l, &tp[next_col], next_col, BDX_ITEM_COL);
-            col_end[next_col] = l->n;
-            next_col++;
+    for (unsigned k = 0; k < s->ntwo && !rc; k++) {
+        bdx_unit *u = &s->units[s->nunits++];
+        const unsigned nrow = (unsigned)BDX_CEIL(tp[k].h, BDX_ROW_UNIT(tp[k].w));
+        const unsigned ncol = (unsigned)BDX_CEIL(tp[k].w, BDX_COL_UNIT(tp[k].h));
+        u->nrow = (unsigned short)nrow;
+        u->ncol = (unsigned short)ncol;
+        u->nrow2 = (unsigned short)nrow;
+        u->pad = 0;
+        for (int f = 0; f < 2 && !rc; f++) {
+            u->item0[f] = lists[f].n;
+            for (unsigned i = 0; i < nrow && !rc; i++)
+                rc = push(&lists[f], tp[k].run, tp[k].idx, i, BDX_ITEM_ROW, k);
+            for (unsigned i = 0; i < ncol && !rc; i++)
+                rc = push(&lists[f], tp[k].run, tp[k].idx, i, BDX_ITEM_COL, k);
+            for (unsigned i = 0; i < nrow && f == 1 && !rc; i++)
+                rc = push(&lists[f], tp[k].run, tp[k].idx, i, BDX_ITEM_ROW2, k);
         }
-        while (!rc && fused && next_row2 < next_col && l->n >= col_end[next_row2] + lag) {
-            rc = push_units(l, &tp[next_row2], next_row2, BDX_ITEM_ROW2);
-            next_row2++;
-        }
-        if (!rc)
-            rc = push_units(l, &tp[k], k, BDX_ITEM_ROW);
-        row_end[k] = l->n;
     }
-    /* whole arrays: independent fillers before the last dependent items */
-    for (unsigned r = 0; r < nruns && !rc; r++) {
-        const int h = (int)runs[r].h, w = (int)runs[r].w;
+    for (unsigned r = 0; r < s->nruns && !rc; r++) {
+        const int h = (int)s->runs[r].h, w = (int)s->runs[r].w;
         if (!BDX_BATCH_SMALL(h, w))
             continue;
-        const unsigned per = (unsigned)(tiny ? BDX_TINY_PER(h, w) : BDX_BATCH_PER(h, w)), n = runs[r].count;
+        const unsigned per = (unsigned)(s->tiny ? BDX_TINY_PER(h, w) : BDX_BATCH_PER(h, w)), n = s->runs[r].count;
         const unsigned parts = (n + per - 1) / per;
         for (unsigned p = 0, first = 0; p < parts && !rc; p++) {
             const unsigned take = (n - first) / (parts - p) + ((n - first) % (parts - p) != 0);
-            rc = push(l, r, first, take, BDX_ITEM_SMALL, 0);
+            bdx_unit *u = &s->units[s->nunits++];
+            u->nrow = 1;
+            u->ncol = 0;
+            u->nrow2 = 0;
+            u->pad = 0;
+            for (int f = 0; f < 2 && !rc; f++) {
+                u->item0[f] = lists[f].n;
+                rc = push(&lists[f], r, first, take, BDX_ITEM_SMALL, 0);
+            }
             first += take;
         }
     }
-    while (!rc && next_col < ntwo) {
-        rc = push_units(l, &tp[next_col], next_col, BDX_ITEM_COL);
-        col_end[next_col] = l->n;
-        next_col++;
-    }
-    while (!rc && fused && next_row2 < ntwo) {
-        rc = push_units(l, &tp[next_row2], next_row2, BDX_ITEM_ROW2);
-        next_row2++;
-    }
-    free(row_end);
-    free(col_end);
     return rc;
 }
 
@@ -231,7 +238,7 @@ void bridct_batch_destroy(bridct_batch *b)
     if (b->graph)
         bdx_graph_destroy(b->graph);
     stage_destroy(b->stage);
-    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_ctr,
+    void *bufs[] = {b->d_tables, b->d_runs, b->d_items[0], b->d_items[1], b->d_units, b->d_ctr,
                     b->d_in,     b->d_out,  b->d_goffs,    b->d_dense,    b->d_dwork};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
@@ -357,6 +364,7 @@ void bdx_schedule_free(bdx_schedule *s)
     free(s->runs);
     free(s->items[0]);
     free(s->items[1]);
+    free(s->units);
     free(s->offsets);
     memset(s, 0, sizeof(*s));
 }
@@ -369,7 +377,7 @@ int bdx_batch_is_tiny(size_t count, const int *heights, const int *widths)
     return count > 0;
 }
 
-int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s)
+int bdx_batch_schedule(size_t count, const int *heights, const int *widths, bdx_schedule *s)
 {
     memset(s, 0, sizeof(*s));
     if (count == 0 || count > UINT32_MAX / 2 || !heights || !widths)
@@ -414,14 +422,14 @@ int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsi
     s->elements = off;
     qsort(tp, s->ntwo, sizeof(two_pass), by_size_desc);
     s->tiny = bdx_batch_is_tiny(count, heights, widths);
-    const int rc = build_items(&lists[0], s->runs, s->nruns, tp, s->ntwo, 0, lag, s->tiny) ||
-                   build_items(&lists[1], s->runs, s->nruns, tp, s->ntwo, 1, lag, s->tiny);
+    const int rc = build_units(s, tp, lists);
     free(tp);
     s->items[0] = lists[0].items;
     s->items[1] = lists[1].items;
     s->nitems[0] = lists[0].n;
     s->nitems[1] = lists[1].n;
-    s->nctr = BDX_CTR_BASE + 2 * s->ntwo;
+    s->qbase = BDX_QBASE(s->ntwo);
+    s->nctr = s->qbase + BDX_QWORDS(s->nunits);
     if (rc) {
         bdx_schedule_free(s);
         return -1;
@@ -444,7 +452,7 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     b->blocks = 2u * (unsigned)cus;
     b->spin_limit = SPIN_LIMIT;
     bdx_schedule sc;
-    if (bdx_batch_schedule(count, heights, widths, 3 * b->blocks, &sc)) {
+    if (bdx_batch_schedule(count, heights, widths, &sc)) {
         free(b);
         return NULL;
     }
@@ -457,6 +465,8 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
     b->nitems[0] = sc.nitems[0];
     b->nitems[1] = sc.nitems[1];
     b->nctr = sc.nctr;
+    b->nunits = sc.nunits;
+    b->qbase = sc.qbase;
     b->tiny = sc.tiny;
     float *host_tables = malloc(BDX_TABLE_FLOATS * sizeof(float));
     unsigned *zeros = calloc(b->nctr, sizeof(unsigned));   /* counters start at zero */
@@ -475,6 +485,7 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         upload((void **)&b->d_runs, sc.runs, sc.nruns * sizeof(bdx_run), b->stream) ||
         upload((void **)&b->d_items[0], sc.items[0], sc.nitems[0] * sizeof(bdx_item), b->stream) ||
         upload((void **)&b->d_items[1], sc.items[1], sc.nitems[1] * sizeof(bdx_item), b->stream) ||
+        upload((void **)&b->d_units, sc.units, sc.nunits * sizeof(bdx_unit), b->stream) ||
         upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) ||
         upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) ||
         bdx_stream_create(&b->aux_stream) || bdx_event_create(&b->ev_fork) || bdx_event_create(&b->ev_join) ||
@@ -518,8 +529,9 @@ static int launch(bridct_batch *b, int mode, const float *d_in, float *d_out)
     int flags = 0;
     if (b->tiny)
         flags = BDX_LAUNCH_TINY | (2 * b->elements * sizeof(float) > BDX_STREAM_MIN_BYTES ? BDX_LAUNCH_STREAM : 0);
-    return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_ctr, b->nctr,
-                            mode, b->d_tables, b->blocks, b->spin_limit, flags, b->stream);
+    return bdx_batch_launch(d_in, d_out, b->d_runs, b->d_items[fused], b->nitems[fused], b->d_units, b->nunits,
+                            b->qbase, b->d_ctr, b->nctr, mode, b->d_tables, b->blocks, b->spin_limit, flags,
+                            b->stream);
 }
 
 /* A dense shape on the batch stream, in slices of the GEMMs' grid limits; the round trip

@@ -2,20 +2,30 @@
 // launch.  Compiled once per transform mode BDX_MODE (0 forward, 1 inverse, 2 fused
 // round trip).
 //
-// BDX_BATCH_THREADS-lane workgroups, as many as stay resident, take work items from one
-// queue (an atomic head counter) until it is empty.  Items (built on the host, see
-// bdx_item in bridct_device.h) are:
+// BDX_BATCH_THREADS-lane workgroups, as many as stay resident, take work items until the
+// batch is done.  Items (built on the host, see bdx_item in bridct_device.h) are:
 //   SMALL  several whole arrays of one run (small_unit: one HBM read and write per pixel);
 //   ROW    a band of rows of a larger array, input -> output (row_unit);
 //   COL    a strip of columns of that array, in place on the output (col_unit); it waits
 //          until every ROW item of the array has completed;
 //   ROW2   (round trip only) the inverse row band, in place, after every COL item.
-// The queue lists all ROW items of an array before its COL items (and those before its
-// ROW2 items), so a waiting workgroup only waits for items already taken by running
-// workgroups, which never wait on anything later: the schedule cannot deadlock.  The
-// host interleaves the arrays so that the columns of one array run while the rows of the
-// next are in flight, which keeps the intermediate in the caches.  A wait longer than
-// spin_limit sleeps sets the error flag and proceeds instead of hanging.
+// Units (bdx_unit): one two-pass array with all its items, or one SMALL item.
+//
+// Queues per XCD.  A workgroup reads the id of its XCD and draws tickets from that XCD's
+// queue.  The tickets of a queue are cut into slots: slot j is the ROW items of the unit
+// u_j the queue takes j-th, then the COL items of u_{j-1}, then (round trip) the ROW2
+// items of u_{j-2}.  The workgroup that draws the first ticket of slot j takes u_j from the
+// global unit counter and publishes the slot (unit and end ticket, one 64-bit word);
+// workgroups with later tickets of the slot wait for that.  So every item of an array runs
+// on one XCD: the row pass's output stays in that XCD's L2 for the column pass, and a
+// dependency needs no cache write-back, only that the producer's stores have reached
+// the L2 before its counter moves, and that the consumer's CU drops its L1 lines (a CU may
+// hold a line of the array from before the producers wrote it) before reading: an
+// agent-scope acquire, no whole-L2 write-back on the producer's side.  Every wait
+// is for items of earlier tickets of the same queue, taken by running workgroups, which
+// never wait on later ones: the schedule cannot deadlock.  A wait longer than spin_limit
+// sleeps sets the error flag and proceeds instead of hanging.  The columns of u_{j-1} are
+// drawn after the rows of u_j, so they seldom wait.
 #include "bdx_units.h"
 
 #ifndef BDX_MODE
@@ -124,9 +134,11 @@ __device__ __forceinline__ void col_any(int h, int w, const float *in, float *ou
     }
 }
 
-// Waits until *p >= target: thread 0 spins, then acquires at agent scope (invalidates the
-// stale lines of this CU's L1 and this XCD's L2: producers may sit on other XCDs, whose L2
-// caches are not coherent with this one), and the block follows through the barrier.
+// Waits until *p >= target: thread 0 spins, then acquires at agent scope, which invalidates
+// this CU's L1 (a CU that ran a column strip of the array holds whole lines of it, the
+// neighbouring strips' columns included, from before their producers wrote them; an
+// invalidation at workgroup scope left such lines in place), and the block follows through
+// the barrier.  The producers are on this XCD, so its L2 already holds their stores.
 __device__ __forceinline__ void wait_for(unsigned *p, unsigned target, unsigned *err, unsigned limit)
 {
     if (threadIdx.x == 0) {
@@ -144,35 +156,138 @@ __device__ __forceinline__ void wait_for(unsigned *p, unsigned target, unsigned 
     __syncthreads();
 }
 
-// Publishes the block's global stores at agent scope, then counts the item as done: every
-// wave waits until its stores have reached the L2, then thread 0 releases (writes the
-// whole L2 of its XCD back, which covers all waves' stores), waits for the write-back and
-// increments the counter.
+// Counts the item as done once the block's stores have reached the L2 of this XCD: every
+// wave waits for its stores, then thread 0 increments the counter.
 __device__ __forceinline__ void signal_done(unsigned *p)
 {
     wait_vm();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        wait_vm();
+    if (threadIdx.x == 0)
         __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
+// Id of the XCD that runs this workgroup (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ unsigned xcd_id()
+{
+    unsigned v;
+    __asm__ volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & (BDX_XQ - 1);
+}
+
+// Items of slot j: the rows of u_j, the columns of u_{j-L} and (round trip) the ROW2 items
+// of u_{j-2L}, L = BDX_LAG.
+__device__ __forceinline__ unsigned slot_items(const bdx_unit *__restrict__ units, unsigned u0, unsigned u1,
+                                               unsigned u2)
+{
+    return (u0 != BDX_UNIT_END ? units[u0].nrow : 0u) + (u1 != BDX_UNIT_END ? units[u1].ncol : 0u) +
+           (BDX_MODE == 2 && u2 != BDX_UNIT_END ? units[u2].nrow2 : 0u);
+}
+
+// Units of the last K + 1 slots of a queue walk, newest first (K = 2 BDX_LAG).
+struct SlotHistory {
+    static constexpr int K = 2 * BDX_LAG;
+    unsigned u[K + 1];
+    __device__ __forceinline__ void init()
+    {
+#pragma unroll
+        for (int i = 0; i <= K; i++)
+            u[i] = BDX_UNIT_END;
+    }
+    __device__ __forceinline__ void push(unsigned v)
+    {
+#pragma unroll
+        for (int i = K; i > 0; i--)
+            u[i] = u[i - 1];
+        u[0] = v;
+    }
+    // undoes push (the walk stays at this slot for the next ticket)
+    __device__ __forceinline__ void pop()
+    {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+            u[i] = u[i + 1];
+        u[K] = BDX_UNIT_END;
+    }
+    // this slot and every later one are empty: the units of the slots it and they draw
+    // columns or ROW2 items from are all exhausted
+    __device__ __forceinline__ bool terminal() const
+    {
+        bool t = true;
+#pragma unroll
+        for (int i = 0; i <= (BDX_MODE == 2 ? K : BDX_LAG); i++)
+            t = t && u[i] == BDX_UNIT_END;
+        return t;
+    }
+};
+
 __global__ __launch_bounds__(TH, 2) void k_batch(const float *in, float *out, const bdx_run *__restrict__ runs,
-                                              const bdx_item *__restrict__ items, unsigned nitems,
-                                              unsigned *ctr, const float *__restrict__ tab, unsigned spin_limit)
+                                              const bdx_item *__restrict__ items, const bdx_unit *__restrict__ units,
+                                              unsigned nunits, unsigned qbase, unsigned *ctr,
+                                              const float *__restrict__ tab, unsigned spin_limit)
 {
     __shared__ float smem[BDX_BATCH_LDS];
     __shared__ unsigned s_item;
+    constexpr int F = BDX_MODE == 2 ? 1 : 0;   // item list family
     unsigned *err = ctr + BDX_CTR_ERROR;
+    const unsigned q = xcd_id();
+    unsigned *ticket = ctr + qbase + q, *unit_next = ctr + qbase + BDX_XQ;
+    unsigned long long *slots = reinterpret_cast<unsigned long long *>(ctr + BDX_QSLOT(qbase, nunits, q, 0));
+    // thread 0's walk over the slots of this queue: slot j starts at ticket `start`; hist.u[i]
+    // is the unit of slot j - 1 - i
+    unsigned j = 0, start = 0;
+    SlotHistory hist;
+    hist.init();
     for (;;) {
-        if (threadIdx.x == 0)
-            s_item = __hip_atomic_fetch_add(ctr + BDX_CTR_HEAD, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) {
+            const unsigned v = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned it = ~0u;   // item to run, or ~0u: no more work on this queue
+            for (unsigned n = 0; j < BDX_SLOTS(nunits);) {
+                unsigned long long w = __hip_atomic_load(slots + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (w == 0 && v == start) {   // first ticket of slot j: take its unit and publish it
+                    unsigned u = __hip_atomic_fetch_add(unit_next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    u = u < nunits ? u : BDX_UNIT_END;
+                    w = (unsigned long long)(u == BDX_UNIT_END ? BDX_UNIT_END : u + 1u) << 32 |
+                        (start + slot_items(units, u, hist.u[BDX_LAG - 1], hist.u[2 * BDX_LAG - 1]));
+                    __hip_atomic_store(slots + j, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                while (w == 0) {   // published by the holder of its first ticket, drawn earlier
+                    if (++n > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    w = __hip_atomic_load(slots + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (w == 0)
+                    break;   // error flag set: give up
+                const unsigned hi = (unsigned)(w >> 32), end = (unsigned)w;
+                const unsigned u0 = hi == BDX_UNIT_END ? BDX_UNIT_END : hi - 1u;
+                const unsigned u1 = hist.u[BDX_LAG - 1], u2 = hist.u[2 * BDX_LAG - 1];
+                hist.push(u0);
+                if (hist.terminal())
+                    break;   // this and every later slot are empty
+                if (v < end) {
+                    unsigned o = v - start;
+                    const unsigned r0 = u0 != BDX_UNIT_END ? units[u0].nrow : 0u;
+                    const unsigned c1 = u1 != BDX_UNIT_END ? units[u1].ncol : 0u;
+                    if (o < r0)
+                        it = units[u0].item0[F] + o;
+                    else if ((o -= r0) < c1)
+                        it = units[u1].item0[F] + units[u1].nrow + o;
+                    else
+                        it = units[u2].item0[F] + units[u2].nrow + units[u2].ncol + (o - c1);
+                    hist.pop();
+                    break;
+                }
+                start = end;
+                j++;
+            }
+            s_item = it;
+        }
         __syncthreads();
         const unsigned i = s_item;
         __syncthreads();   // every lane has read s_item before thread 0 writes it again
-        if (i >= nitems)
+        if (i == ~0u)
             break;
         const bdx_item it = items[i];
         const bdx_run rn = runs[it.run];
@@ -254,7 +369,8 @@ __global__ __launch_bounds__(TH) void k_batch_tiny(const float *in, float *out, 
 #define BDX_CAT(a, b) BDX_CAT2(a, b)
 
 int BDX_CAT(launch_batch_m, BDX_MODE)(int flags, const float *in, float *out, const bdx_run *runs,
-                                      const bdx_item *items, unsigned nitems, unsigned *ctr, const float *tab,
+                                      const bdx_item *items, unsigned nitems, const bdx_unit *units,
+                                      unsigned nunits, unsigned qbase, unsigned *ctr, const float *tab,
                                       unsigned blocks, unsigned spin_limit, hipStream_t s)
 {
     if ((flags & BDX_LAUNCH_TINY) && (flags & BDX_LAUNCH_STREAM))
@@ -262,8 +378,8 @@ int BDX_CAT(launch_batch_m, BDX_MODE)(int flags, const float *in, float *out, co
     else if (flags & BDX_LAUNCH_TINY)
         hipLaunchKernelGGL(k_batch_tiny<false>, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
     else
-        hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, nitems, ctr, tab,
-                           spin_limit);
+        hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, units, nunits, qbase, ctr,
+                           tab, spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -57,10 +57,13 @@ int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mod
 
 // Persistent batch kernel, one per mode (bdx_batch.hip); flags: BDX_LAUNCH_*.
 int launch_batch_m0(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
-                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
+                    const bdx_unit *units, unsigned nunits, unsigned qbase, unsigned *ctr, const float *tab,
+                    unsigned blocks, unsigned spin_limit, hipStream_t s);
 int launch_batch_m1(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
-                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
+                    const bdx_unit *units, unsigned nunits, unsigned qbase, unsigned *ctr, const float *tab,
+                    unsigned blocks, unsigned spin_limit, hipStream_t s);
 int launch_batch_m2(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,
-                    unsigned *ctr, const float *tab, unsigned blocks, unsigned spin_limit, hipStream_t s);
+                    const bdx_unit *units, unsigned nunits, unsigned qbase, unsigned *ctr, const float *tab,
+                    unsigned blocks, unsigned spin_limit, hipStream_t s);
 
 }  // namespace bdx

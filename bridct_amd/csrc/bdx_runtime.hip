@@ -212,8 +212,9 @@ int bdx_device_cus(void)
 }
 
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
-                     unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, int flags, void *stream)
+                     unsigned nitems, const bdx_unit *d_units, unsigned nunits, unsigned qbase,
+                     unsigned *d_counters, unsigned ncounters, int mode, const float *d_tables, unsigned blocks,
+                     unsigned spin_limit, int flags, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     // a fill kernel, not a memset node, so that the sequence can be captured in a graph; the
@@ -222,13 +223,18 @@ int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, con
         hipLaunchKernelGGL(k_zero, dim3((ncounters + 255) / 256), dim3(256), 0, s, d_counters, ncounters);
     if (hipGetLastError() != hipSuccess || nitems == 0)
         return nitems == 0 ? 0 : -1;
+    if (!(flags & BDX_LAUNCH_TINY) && (qbase % 2 || qbase + BDX_QWORDS(nunits) > ncounters))
+        return -1;
     switch (mode) {
     case BDX_FORWARD:
-        return bdx::launch_batch_m0(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m0(flags, d_in, d_out, d_runs, d_items, nitems, d_units, nunits, qbase, d_counters,
+                                    d_tables, blocks, spin_limit, s);
     case BDX_INVERSE:
-        return bdx::launch_batch_m1(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m1(flags, d_in, d_out, d_runs, d_items, nitems, d_units, nunits, qbase, d_counters,
+                                    d_tables, blocks, spin_limit, s);
     case BDX_ROUNDTRIP:
-        return bdx::launch_batch_m2(flags, d_in, d_out, d_runs, d_items, nitems, d_counters, d_tables, blocks, spin_limit, s);
+        return bdx::launch_batch_m2(flags, d_in, d_out, d_runs, d_items, nitems, d_units, nunits, qbase, d_counters,
+                                    d_tables, blocks, spin_limit, s);
     default:
         return -1;
     }

@@ -66,11 +66,27 @@ extern "C" {
 #define BDX_ITEM_ROW 1u
 #define BDX_ITEM_COL 2u
 #define BDX_ITEM_ROW2 3u
-/* counters block: [0] queue head, [1] error flag, then per two-pass array
- * [2 + 2 c] row units done, [3 + 2 c] column units done */
+/* counters block: [0] unused, [1] error flag, then per two-pass array c
+ * [2 + 2 c] row items done, [3 + 2 c] column items done, then (from BDX_QBASE(ntwo), an even
+ * word offset) the per-XCD queues of the persistent kernel:
+ *   BDX_XQ ticket counters, one global unit counter, one pad word, then per queue q
+ *   BDX_SLOTS(nunits) 64-bit slot words (slot j of queue q at word
+ *   BDX_QSLOT(base, nunits, q, j)): 0 while unpublished, else (unit + 1) << 32 | end ticket,
+ *   unit + 1 = BDX_UNIT_END when the units are exhausted. */
 #define BDX_CTR_HEAD 0
 #define BDX_CTR_ERROR 1
 #define BDX_CTR_BASE 2
+#define BDX_XQ 16                               /* queues, indexed by the XCD id (4 bits) */
+#define BDX_QBASE(ntwo) ((BDX_CTR_BASE + 2u * (ntwo) + 1u) & ~1u)
+/* Lag of the queues, in slots: slot j holds the ROW items of the unit u_j a queue takes
+ * j-th, the COL items of u_{j-BDX_LAG} and the ROW2 items of u_{j-2 BDX_LAG}. */
+#ifndef BDX_LAG
+#define BDX_LAG 3
+#endif
+#define BDX_SLOTS(nunits) ((nunits) + 2u * BDX_LAG + 1u)   /* slots a queue can reach */
+#define BDX_QSLOT(base, nunits, q, j) ((base) + BDX_XQ + 2u + 2u * ((q) * BDX_SLOTS(nunits) + (j)))
+#define BDX_QWORDS(nunits) (BDX_XQ + 2u + 2u * BDX_XQ * BDX_SLOTS(nunits))
+#define BDX_UNIT_END 0xFFFFFFFFu
 
 typedef struct {
     unsigned h, w, count, pad;
@@ -84,22 +100,35 @@ typedef struct {
     unsigned run, idx, unit, kind_ctr;
 } bdx_item;
 
-/* Launches the persistent batch kernel (after a counter reset) on `stream`:
- * `blocks` resident workgroups pull `nitems` items.  spin_limit bounds each dependency
- * wait (a wait that runs out sets the error flag instead of hanging). */
+/* A unit of the persistent kernel: one two-pass array (its ROW items, then its COL items,
+ * then for the fused round trip its ROW2 items) or one SMALL item.  Its items are
+ * consecutive in the item list of each mode family from item0[family] on (family 0:
+ * forward / inverse, 1: fused round trip); nrow2 = nrow for two-pass arrays, 0 for SMALL. */
+typedef struct {
+    unsigned item0[2];
+    unsigned short nrow, ncol, nrow2, pad;
+} bdx_unit;
+
+/* Launches the persistent batch kernel (after a counter reset) on `stream`: `blocks`
+ * resident workgroups take the `nunits` units through the per-XCD queues (every item of a
+ * unit runs on the XCD whose queue took it, so dependencies stay inside one L2).
+ * spin_limit bounds each wait (a wait that runs out sets the error flag instead of
+ * hanging). */
 /* flags of bdx_batch_launch: the tiny kernel instead of the persistent one, and streaming
  * (non-temporal) array accesses in it */
 #define BDX_LAUNCH_TINY 1
 #define BDX_LAUNCH_STREAM 2
 int bdx_batch_launch(const float *d_in, float *d_out, const bdx_run *d_runs, const bdx_item *d_items,
-                     unsigned nitems, unsigned *d_counters, unsigned ncounters, int mode,
-                     const float *d_tables, unsigned blocks, unsigned spin_limit, int flags, void *stream);
+                     unsigned nitems, const bdx_unit *d_units, unsigned nunits, unsigned qbase,
+                     unsigned *d_counters, unsigned ncounters, int mode, const float *d_tables, unsigned blocks,
+                     unsigned spin_limit, int flags, void *stream);
 int bdx_device_cus(void);
 
 /* Host-side schedule of a batch (no device needed): runs of consecutive equal shapes,
- * the item queues ([0] forward / inverse, [1] fused round trip) and the counter count.
- * `lag` = queue distance between an array's last row item and its first column item.
- * Returns 0, or -1 for an invalid shape list or allocation failure. */
+ * the units (two-pass arrays by decreasing size, then the SMALL items) and their item lists
+ * ([0] forward / inverse, [1] fused round trip), and the counter words (per-array
+ * counters and the queues).  Returns 0, or -1 for an invalid shape list or allocation
+ * failure. */
 typedef struct {
     size_t count, elements;
     size_t *offsets;
@@ -107,10 +136,13 @@ typedef struct {
     unsigned nruns, ntwo;
     bdx_item *items[2];
     unsigned nitems[2];
-    unsigned nctr;
+    bdx_unit *units;
+    unsigned nunits;
+    unsigned qbase;           /* word offset of the queues in the counters block */
+    unsigned nctr;            /* counter words in all */
     int tiny;                 /* every array fits the tiny kernel; SMALL items hold BDX_TINY_PER */
 } bdx_schedule;
-int bdx_batch_schedule(size_t count, const int *heights, const int *widths, unsigned lag, bdx_schedule *s);
+int bdx_batch_schedule(size_t count, const int *heights, const int *widths, bdx_schedule *s);
 /* 1 when every array of the shape list fits the tiny kernel (both axes <= BDX_TINY_MAX);
  * its schedule then packs BDX_TINY_PER(h, w) arrays per item at most. */
 int bdx_batch_is_tiny(size_t count, const int *heights, const int *widths);
