@@ -34,9 +34,8 @@ extern "C" {
 #define BDX_ROW_BAND(w) ((w) >= 128 ? BDX_BATCH_THREADS / BDX_TEAM(w) : BDX_BATCH_THREADS)
 #define BDX_COL_STRIP(h) ((h) >= 128 ? BDX_BATCH_THREADS / BDX_TEAM(h) : BDX_BATCH_THREADS)
 /* one work item: BDX_ITEM_BANDS bands / strips for lines of 512 and more, else one.
- * Larger items mean fewer cache write-backs and invalidations per byte (measured: 64
- * arrays of 1024^2 gain 6% at 2 and 30% at 4) but fewer items per array, which costs the
- * ragged batches more (10,000 random shapes: 3% slower at 2, 20% at 4): one. */
+ * Larger items mean fewer dependency waits per byte but fewer items per array; on the
+ * 10,000-array ragged mix two bands per item measured 1-2% slower than one: one. */
 #ifndef BDX_ITEM_BANDS
 #define BDX_ITEM_BANDS 1
 #endif
