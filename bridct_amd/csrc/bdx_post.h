@@ -52,9 +52,13 @@ __device__ __forceinline__ void post_inv(const float (&o)[4], float wr, float wi
     bi = orr - ei;
 }
 
+// The sums are never contracted with the products that scale o (inputs times normalization
+// factors): whether the compiler fused them would depend on the block the inputs were read
+// in, and kernels that read in[0], in[N] apart must give the same bits as those that do not.
 __device__ __forceinline__ void post_inv_special(const float (&o)[4], float th, float &z0r, float &z0i,
                                                  float &zhr, float &zhi)
 {
+#pragma clang fp contract(off)
     z0r = o[0] + o[1];
     z0i = o[0] - o[1];
     zhr = __builtin_fmaf(th, o[3], o[2]);

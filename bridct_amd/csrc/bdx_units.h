@@ -157,25 +157,43 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         // offsets fit 32 bits).
         const unsigned w4 = 4u * (unsigned)w, l4 = 4u * (unsigned)line;
         const int row_p = 4 * t, row_r = 4 * (T - 1 - t), row_c = t, row_d = T - 1 - t;
+        // Low slots of the lead member (several members per wave, t = 0): lane part N1/2.  The
+        // DL part can be negative, so it is kept biased by BL rows (and its uniform rows
+        // lowered by as many): every offset stays a non-negative 32-bit value.
+        constexpr int BL = C::N1 / 2;
+        const int row_cl = M > 1 && t == 0 ? C::N1 / 2 : t, row_dl = T - 1 - row_cl + BL;
         const unsigned off_p = (unsigned)row_p * w4 + l4, off_r = (unsigned)row_r * w4 + l4;
         const unsigned off_c = (unsigned)row_c * w4 + l4, off_d = (unsigned)row_d * w4 + l4;
+        const unsigned off_cl = (unsigned)row_cl * w4 + l4, off_dl = (unsigned)row_dl * w4 + l4;
         const auto off = [=](int fam) {
-            return fam == FAM_P ? off_p : fam == FAM_R ? off_r : fam == FAM_C ? off_c : off_d;
+            return fam == FAM_P ? off_p
+                 : fam == FAM_R ? off_r
+                 : fam == FAM_C ? off_c
+                 : fam == FAM_D ? off_d
+                 : fam == FAM_CL ? off_cl
+                                 : off_dl;
         };
+        const auto urow_of = [](int fam, int urow) { return (ptrdiff_t)(fam == FAM_DL ? urow - BL : urow); };
         const char *src = reinterpret_cast<const char *>(in + base + c0);
         char *dst = reinterpret_cast<char *>(out + base + c0);
         const auto ld = [&](int fam, int urow) {
-            return live ? ld1<NT>(reinterpret_cast<const float *>(src + (size_t)urow * w4 + off(fam))) : 0.0f;
+            return live ? ld1<NT>(reinterpret_cast<const float *>(src + urow_of(fam, urow) * w4 + off(fam))) : 0.0f;
         };
         const auto st = [&](int fam, int urow, float v) {
             if (live)
-                st1<NT>(reinterpret_cast<float *>(dst + (size_t)urow * w4 + off(fam)), v);
+                st1<NT>(reinterpret_cast<float *>(dst + urow_of(fam, urow) * w4 + off(fam)), v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
         // one pointer per row family, so that each factor is that pointer plus an immediate
         const float *al_p = al + row_p, *al_r = al + row_r, *al_c = al + row_c, *al_d = al + row_d;
+        const float *al_cl = al + row_cl, *al_dl = al + row_dl;
         const auto alpha = [&](int fam, int urow) {
-            return (fam == FAM_P ? al_p : fam == FAM_R ? al_r : fam == FAM_C ? al_c : al_d)[urow];
+            return (fam == FAM_P    ? al_p
+                    : fam == FAM_R  ? al_r
+                    : fam == FAM_C  ? al_c
+                    : fam == FAM_D  ? al_d
+                    : fam == FAM_CL ? al_cl
+                                    : al_dl)[urow_of(fam, urow)];
         };
         const CConst<H, M> cc(tab + BDX_TABLE_BASE(H), lt, t);
         if constexpr (MODE == 1) {
@@ -187,11 +205,17 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
                 st(fam, urow, v * alpha(fam, urow));
             });
         } else {
-            float o[Q][T][4];
+            // outputs 4 and 5 (the lead's rows 0 and N with several members per wave) in ex
+            float o[Q][T][4], ex[2] = {0.f, 0.f};
             cteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int r, int k2, int c, int fam, int urow, float v) {
-                o[r][k2][c] = v * alpha(fam, urow);
+                if (c >= 4)
+                    ex[c - 4] = v * alpha(fam, urow);
+                else
+                    o[r][k2][c] = v * alpha(fam, urow);
             });
-            cteam_inv<H, THREADS>(t, line, [&](int r, int k2, int c, int, int) { return o[r][k2][c]; }, st, xl, cc);
+            cteam_inv<H, THREADS>(
+                t, line, [&](int r, int k2, int c, int, int) { return c >= 4 ? ex[c - 4] : o[r][k2][c]; }, st, xl,
+                cc);
         }
     } else {
         // one lane per column: the generated Shao-Johnson core on H values in registers

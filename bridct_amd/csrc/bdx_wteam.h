@@ -113,13 +113,19 @@ template <int n, int M> struct CConst {
     __device__ __forceinline__ float tn(int k) const { return M == 1 ? g[2 * n + k] : l[R::TN + k]; }
 };
 
-// Row families (lane parts): 0 = P (4t), 1 = R (4(T-1-t)), 2 = C (t), 3 = D (T-1-t).
-enum { FAM_P = 0, FAM_R = 1, FAM_C = 2, FAM_D = 3 };
+// Row families (lane parts): 0 = P (4t), 1 = R (4(T-1-t)), 2 = C (t), 3 = D (T-1-t), and
+// the low slots of the lead member with several members per wave: 4 = CL (tl), 5 = DL
+// (T-1-tl), tl = N1/2 for t = 0, else t.
+enum { FAM_P = 0, FAM_R = 1, FAM_C = 2, FAM_D = 3, FAM_CL = 4, FAM_DL = 5 };
 
 // Family and uniform part of output row (k2, which) of pair group g = t + T r, g != 0.
 template <int n, int T> __device__ __forceinline__ constexpr int out_family(int which)
 {
     return which == 1 || which == 2 ? FAM_D : FAM_C;
+}
+template <int n, int T> __device__ __forceinline__ constexpr int out_family_lo(int which)
+{
+    return which == 1 || which == 2 ? FAM_DL : FAM_CL;
 }
 template <int n, int T> __device__ __forceinline__ constexpr int out_urow(int r, int k2, int which)
 {
@@ -249,6 +255,55 @@ __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CCo
     }
     cteam_exchange<n, THREADS, true>(t, line, re, im, xl);
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (C::M > 1) {
+        // Several members per wave: the lead (t = 0) takes the general slot formula too, so
+        // that its wave does not run a second, divergent post-process.  Its low slots (r = 0,
+        // k2 < T/2) take k = N1/2 + N1 k2 from (D1[k2], D1[T-1-k2]) through the lo offsets
+        // (lane part N1/2 instead of t); its high slots k = N1 k2 from (D0[k2], D0[T-k2]),
+        // the pair (N - k, k) seen from its other side, which post_fwd maps to the same four
+        // outputs bitwise; its slot T/2 (k = N/2, rows n/4 and 3n/4, twice each) carries
+        // out[n/4] and out[3n/4] of Z_0 and Z_{N/2}, and out[0], out[N] go out apart
+        // (emit outputs 4 and 5).
+        const bool lead = t == 0;
+        const CConst<n, C::M> cl(cc.g, cc.l, lead ? N1 / 2 : t);
+        float lead_first = 0.f, lead_second = 0.f;
+#pragma unroll
+        for (int r = 0; r < Q; r++) {
+            float (&d0r)[T] = sub<T>(re, 2 * r), (&d0i)[T] = sub<T>(im, 2 * r);
+            float (&d1r)[T] = sub<T>(re, 2 * r + 1), (&d1i)[T] = sub<T>(im, 2 * r + 1);
+            Dft<T, -1>::run(d0r, d0i);
+            Dft<T, -1>::run(d1r, d1i);
+#pragma unroll
+            for (int k2 = 0; k2 < T; k2++) {
+                const bool lo = r == 0 && k2 < T / 2, hi_swap = r == 0 && k2 > T / 2;
+                const CConst<n, C::M> &P = lo ? cl : cc;
+                const float ar = lo && lead ? d1r[k2] : d0r[k2];
+                const float ai = lo && lead ? d1i[k2] : d0i[k2];
+                const float br = hi_swap && lead ? d0r[hi_swap ? T - k2 : 0] : d1r[T - 1 - k2];
+                const float bi = hi_swap && lead ? d0i[hi_swap ? T - k2 : 0] : d1i[T - 1 - k2];
+                float o[4];
+                post_fwd(ar, ai, br, bi, P.oms_re(r, k2), P.oms_im(r, k2), P.tns(r, k2), P.tnns(r, k2), o);
+                if (r == 0 && k2 == T / 2) {
+                    float sp[4];
+                    post_fwd_special(d0r[0], d0i[0], d0r[T / 2], d0i[T / 2], cc.tn(n / 4), sp);
+#pragma unroll
+                    for (int w = 0; w < 4; w++)
+                        o[w] = lead ? sp[2 + (w & 1)] : o[w];
+                    lead_first = sp[0];
+                    lead_second = sp[1];
+                }
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    emit(r, k2, w, lo ? out_family_lo<n, T>(w) : out_family<n, T>(w), out_urow<n, T>(r, k2, w),
+                         o[w]);
+            }
+        }
+        if (lead) {
+            emit(0, 0, 4, FAM_C, 0, lead_first);
+            emit(0, 0, 5, FAM_C, N, lead_second);
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < Q; r++) {
         float (&d0r)[T] = sub<T>(re, 2 * r), (&d0i)[T] = sub<T>(im, 2 * r);
@@ -300,11 +355,61 @@ __device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *
     using C = CTeam<n, THREADS>;
     constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q;
     float re[N1], im[N1];
+    // Several members per wave: the slots of cteam_fwd, lead included (in(.., 4 / 5, ..) are
+    // in[0] and in[N], read by the lead only).
+    const bool lead = t == 0;
+    const CConst<n, C::M> cl(cc.g, cc.l, C::M > 1 && lead ? N1 / 2 : t);
+    float lead_in0 = 0.f, lead_inN = 0.f;
+    if (C::M > 1 && lead) {
+        lead_in0 = in(0, 0, 4, FAM_C, 0);
+        lead_inN = in(0, 0, 5, FAM_C, N);
+    }
 #pragma unroll
     for (int r = 0; r < Q; r++) {
         float (&d0r)[T] = sub<T>(re, 2 * r), (&d0i)[T] = sub<T>(im, 2 * r);
         float (&d1r)[T] = sub<T>(re, 2 * r + 1), (&d1i)[T] = sub<T>(im, 2 * r + 1);
-        if (r == 0 && t == 0) {
+        if constexpr (C::M > 1) {
+            float ar[T], ai[T], br[T], bi[T];
+            float z0r = 0.f, z0i = 0.f, zhr = 0.f, zhi = 0.f;
+#pragma unroll
+            for (int k2 = 0; k2 < T; k2++) {
+                const bool lo = r == 0 && k2 < T / 2;
+                const CConst<n, C::M> &P = lo ? cl : cc;
+                float o[4];
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    o[w] = in(r, k2, w, lo ? out_family_lo<n, T>(w) : out_family<n, T>(w), out_urow<n, T>(r, k2, w));
+                post_inv(o, P.oms_re(r, k2), P.oms_im(r, k2), P.tns(r, k2), P.tnns(r, k2), ar[k2], ai[k2], br[k2],
+                         bi[k2]);
+                if (r == 0 && k2 == T / 2) {
+                    const float os[4] = {lead_in0, lead_inN, o[0], o[1]};
+                    post_inv_special(os, cc.tn(n / 4), z0r, z0i, zhr, zhi);
+                }
+            }
+            // D0[x] = Z_{g + N1 x}, D1[x] = Z_{N1 - g + N1 x}: from slot x and the b side of
+            // slot T-1-x; the lead's group 0 takes D0[0] = Z_0, D0[T/2] = Z_{N/2}, D0[x] from
+            // the b side of its high slot T - x (x < T/2), and D1[x] from the a side of its
+            // low slot x (x < T/2).
+            const bool l0 = r == 0 && lead;
+#pragma unroll
+            for (int x = 0; x < T; x++) {
+                float pr = ar[x], pi = ai[x];
+                if (r == 0 && x == 0) {
+                    pr = l0 ? z0r : pr;
+                    pi = l0 ? z0i : pi;
+                } else if (r == 0 && x == T / 2) {
+                    pr = l0 ? zhr : pr;
+                    pi = l0 ? zhi : pi;
+                } else if (r == 0 && x < T / 2) {
+                    pr = l0 ? br[T - x] : pr;
+                    pi = l0 ? bi[T - x] : pi;
+                }
+                d0r[x] = pr;
+                d0i[x] = pi;
+                d1r[x] = x < T / 2 && l0 ? ar[x] : br[T - 1 - x];
+                d1i[x] = x < T / 2 && l0 ? ai[x] : bi[T - 1 - x];
+            }
+        } else if (r == 0 && t == 0) {
 #pragma unroll
             for (int k2 = 0; k2 < T; k2++) {
                 float o[4];
