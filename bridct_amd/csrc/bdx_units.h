@@ -173,7 +173,9 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
                  : fam == FAM_CL ? off_cl
                                  : off_dl;
         };
-        const auto urow_of = [](int fam, int urow) { return (ptrdiff_t)(fam == FAM_DL ? urow - BL : urow); };
+        // (the DL rows a lead reads or writes, N - k and n - k with k < N/2, exceed BL: the
+        // lowered row is still non-negative, and unsigned keeps the scalar-base form)
+        const auto urow_of = [](int fam, int urow) { return (size_t)(unsigned)(fam == FAM_DL ? urow - BL : urow); };
         const char *src = reinterpret_cast<const char *>(in + base + c0);
         char *dst = reinterpret_cast<char *>(out + base + c0);
         const auto ld = [&](int fam, int urow) {
@@ -201,8 +203,9 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
                 t, line, [&](int, int, int, int fam, int urow) { return ld(fam, urow) * alpha(fam, urow); }, st, xl,
                 cc);
         } else if constexpr (MODE == 0) {
-            cteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int, int, int, int fam, int urow, float v) {
-                st(fam, urow, v * alpha(fam, urow));
+            cteam_fwd<H, THREADS>(t, line, ld, xl, cc, [&](int, int, int c, int fam, int urow, float v) {
+                if (c < 4 || t == 0)   // outputs 4 and 5 belong to the lead only
+                    st(fam, urow, v * alpha(fam, urow));
             });
         } else {
             // outputs 4 and 5 (the lead's rows 0 and N with several members per wave) in ex

@@ -18,8 +18,11 @@
 // N1 k2), (T-1-t) + (N - T + 1 - T r - N1 k2) and t + (N + T r + N1 k2) (families C, D, D,
 // C).  So a lane keeps four byte offsets (one per family) and each access adds a uniform
 // row offset: the address is a scalar base plus one of those four registers, with no
-// per-access vector arithmetic.  The member of lanes with t = 0 also owns the self-paired
-// group g = 0, whose rows are absolute (its lane part is 0 in family C).
+// per-access vector arithmetic.  The member t = 0 (the lead) also owns the self-paired
+// groups 0 and N1/2.  With one member per wave it takes a branch of its own for them
+// (absolute rows); with several it follows the general slot formula through two more
+// families (CL, DL: lane part N1/2 instead of t below slot T/2), so that no wave runs a
+// second, divergent post-process.
 //
 // Registers.  The exchange writes a lane's N1 complex values to LDS and reads the values
 // of its pair groups back into the same registers (Q 2 T = N1), so one set of 2 N1 floats
@@ -230,7 +233,9 @@ template <int T> __device__ __forceinline__ float (&sub(float *v, int i))[T]
 
 // Forward core.  ld(family, urow) returns the lane's element of that row; emit(r, k2, which,
 // family, urow, v) receives the unscaled output of that row as soon as it is computed
-// (family FAM_C with the absolute row for the group g = 0).
+// (family FAM_C with the absolute row for the group g = 0).  With several members per wave,
+// which = 4 and 5 carry out[0] and out[N] of the lead (t = 0); they reach emit on every
+// lane, and only the lead's are outputs.
 template <int n, int THREADS, typename Ld, typename Emit>
 __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CConst<n, CTeam<n, THREADS>::M> cc,
                                           Emit emit)
@@ -298,10 +303,10 @@ __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CCo
                          o[w]);
             }
         }
-        if (lead) {
-            emit(0, 0, 4, FAM_C, 0, lead_first);
-            emit(0, 0, 5, FAM_C, N, lead_second);
-        }
+        // on every lane (no divergent branch): emit keeps outputs 4 and 5 of lanes t != 0
+        // out of memory
+        emit(0, 0, 4, FAM_C, 0, lead_first);
+        emit(0, 0, 5, FAM_C, N, lead_second);
         return;
     }
 #pragma unroll
@@ -360,7 +365,10 @@ __device__ __forceinline__ void cteam_inv(int t, int line, In in, St st, float *
     const bool lead = t == 0;
     const CConst<n, C::M> cl(cc.g, cc.l, C::M > 1 && lead ? N1 / 2 : t);
     float lead_in0 = 0.f, lead_inN = 0.f;
-    if (C::M > 1 && lead) {
+    // The whole first wave reads them (a uniform branch): under a lane-divergent one the
+    // compiler built every later row address with 64-bit vector arithmetic instead of a
+    // scalar base plus a 32-bit offset (1634 against 1328 VALU instructions at n = 1024).
+    if (C::M > 1 && __builtin_amdgcn_readfirstlane(t / C::M) == 0) {
         lead_in0 = in(0, 0, 4, FAM_C, 0);
         lead_inN = in(0, 0, 5, FAM_C, N);
     }
