@@ -226,6 +226,15 @@ __device__ __forceinline__ void cteam_exchange(int t, int line, float (&re)[CTea
     }
 }
 
+// i with its log2(N) bits reversed (N a power of two)
+template <int N> __device__ __forceinline__ constexpr int bit_reverse(int i)
+{
+    int r = 0;
+    for (int b = 1; b < N; b <<= 1)
+        r = (r << 1) | ((i & b) ? 1 : 0);
+    return r;
+}
+
 template <int T> __device__ __forceinline__ float (&sub(float *v, int i))[T]
 {
     return *reinterpret_cast<float(*)[T]>(v + i * T);
@@ -243,8 +252,12 @@ __device__ __forceinline__ void cteam_fwd(int t, int line, Ld ld, float *xl, CCo
     using C = CTeam<n, THREADS>;
     constexpr int T = C::T, N = C::N, N1 = C::N1, Q = C::Q;
     float re[N1], im[N1];
+    // Loads in bit-reversed order, the order in which the DFT_N1 codelet takes its inputs
+    // (its first butterflies combine x[0], x[N1/2], x[N1/4], ...): they can start while
+    // the later loads are in flight.
 #pragma unroll
-    for (int j1 = 0; j1 < N1; j1++) {
+    for (int i = 0; i < N1; i++) {
+        const int j1 = bit_reverse<N1>(i);
         const int fam = in_family<n, T>(j1), u = in_urow<n, T>(j1);
         re[j1] = ld(fam, u);
         im[j1] = ld(fam, fam == FAM_P ? u + 2 : u - 2);
