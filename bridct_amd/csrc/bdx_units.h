@@ -142,7 +142,10 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
     if constexpr (H >= 128) {
         using C = CTeam<H, THREADS>;
         constexpr int T = C::T, Q = C::Q, L = C::L, M = C::M;
-        float *xl = smem, *lt = smem + C::XLDS;
+        // constant table first: its offsets then fit the 16-bit immediate of DS
+        // instructions (after the 64 KiB exchange area they did not, and every table read
+        // took an address instruction)
+        float *lt = smem, *xl = smem + (M > 1 ? C::TLDS : 0);
         if (M > 1 && !tables_ready) {
             load_row_tables<H, MODE == 2 ? FA2 : FA, THREADS>(lt, tab);
             __syncthreads();
