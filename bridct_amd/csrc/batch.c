@@ -157,16 +157,6 @@ static int by_size_desc(const void *a, const void *b)
     return x->idx < y->idx ? -1 : x->idx > y->idx;
 }
 
-static int push_units(item_list *l, const two_pass *a, unsigned c, unsigned kind)
-{
-    const unsigned units = kind == BDX_ITEM_COL ? (unsigned)BDX_CEIL(a->w, BDX_COL_UNIT(a->h))
-                                                : (unsigned)BDX_CEIL(a->h, BDX_ROW_UNIT(a->w));
-    for (unsigned u = 0; u < units; u++)
-        if (push(l, a->run, a->idx, u, kind, c))
-            return -1;
-    return 0;
-}
-
 /* Builds the units and the item lists of both mode families ([0] forward / inverse, [1] round
  * trip): each two-pass array (tp, by decreasing size) is a unit of its ROW items, its COL
  * items and, in the round-trip list, its ROW2 items; then each SMALL item (up to
