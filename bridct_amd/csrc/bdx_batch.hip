@@ -44,13 +44,41 @@ constexpr int ROW_OP = BDX_MODE == 0 ? FA : BDX_MODE == 1 ? IA : FA2;
 
 // Each unit instantiation is its own (non-inlined) function, so that it keeps its own
 // register allocation instead of the union of all paths of the dispatch.
+//
+// A callee sees its arguments as per-lane values of unknown address space, so the shared
+// unit code lost what the kernel knew: that arrays and tables are in global memory and
+// the work area in LDS (it emitted FLAT accesses: a 64-bit address per lane, counted by
+// both the memory and the LDS wait counters), and that every argument is the same on all
+// lanes (it built every address with 64-bit vector arithmetic instead of a scalar base).
+// The pointer parameters therefore carry their address space, and each function takes
+// its arguments back through readfirstlane (all lanes of a workgroup pass the same values).
+typedef const float __attribute__((address_space(1))) *gcptr;
+typedef float __attribute__((address_space(1))) *gptr;
+typedef float __attribute__((address_space(3))) *lptr;
+__device__ __forceinline__ gcptr to_g(const float *p) { return (gcptr)p; }
+__device__ __forceinline__ gptr to_g(float *p) { return (gptr)p; }
+__device__ __forceinline__ lptr to_l(float *p) { return (lptr)p; }
+__device__ __forceinline__ unsigned long long uni64(unsigned long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (unsigned long long)hi << 32 | lo;
+}
+__device__ __forceinline__ const float *uni(gcptr p) { return (const float *)(gcptr)uni64((unsigned long long)p); }
+__device__ __forceinline__ float *uni(gptr p) { return (float *)(gptr)uni64((unsigned long long)p); }
+__device__ __forceinline__ float *uni(lptr p)
+{
+    return (float *)(lptr)(unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)(unsigned long long)p);
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ size_t uni(size_t v) { return (size_t)uni64(v); }
+
 template <int H, int W>
-__device__ __noinline__ void small_hw(const float *in, float *out, size_t off, int count, float *smem,
-                                         const float *__restrict__ tab)
+__device__ __noinline__ void small_hw(gcptr in, gptr out, size_t off, int count, lptr smem, gcptr __restrict__ tab)
 {
     if constexpr (BDX_BATCH_SMALL(H, W)) {
         static_assert(BDX_BATCH_PER(H, W) >= 1, "a whole array must fit the LDS area");
-        small_unit<H, W, BDX_MODE, TH>(in, out, off, count, smem, tab);
+        small_unit<H, W, BDX_MODE, TH>(uni(in), uni(out), uni(off), uni(count), uni(smem), uni(tab));
     }
 }
 
@@ -59,14 +87,14 @@ __device__ __forceinline__ void small_h(int w, const float *in, float *out, size
                                         const float *__restrict__ tab)
 {
     switch (w) {
-    case 8: small_hw<H, 8>(in, out, off, count, smem, tab); break;
-    case 16: small_hw<H, 16>(in, out, off, count, smem, tab); break;
-    case 32: small_hw<H, 32>(in, out, off, count, smem, tab); break;
-    case 64: small_hw<H, 64>(in, out, off, count, smem, tab); break;
-    case 128: small_hw<H, 128>(in, out, off, count, smem, tab); break;
-    case 256: small_hw<H, 256>(in, out, off, count, smem, tab); break;
-    case 512: small_hw<H, 512>(in, out, off, count, smem, tab); break;
-    default: small_hw<H, 1024>(in, out, off, count, smem, tab); break;
+    case 8: small_hw<H, 8>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 16: small_hw<H, 16>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 32: small_hw<H, 32>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 64: small_hw<H, 64>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 128: small_hw<H, 128>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 256: small_hw<H, 256>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    case 512: small_hw<H, 512>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
+    default: small_hw<H, 1024>(to_g(in), to_g(out), off, count, to_l(smem), to_g(tab)); break;
     }
 }
 
@@ -86,12 +114,11 @@ __device__ __forceinline__ void small_any(int h, int w, const float *in, float *
 }
 
 template <int W, int OP>
-__device__ __noinline__ void row_w(const float *in, float *out, size_t row0, int nrows, float *smem,
-                                   const float *__restrict__ tab)
+__device__ __noinline__ void row_w(gcptr in, gptr out, size_t row0, int nrows, lptr smem, gcptr __restrict__ tab)
 {
     static_assert(row_unit_lds<W, OP, TH>() <= BDX_BATCH_LDS && row_unit_rows<W, TH>() == BDX_ROW_BAND(W),
                   "row unit geometry");
-    row_unit<W, OP, TH>(in, out, row0, nrows, smem, tab, false);
+    row_unit<W, OP, TH>(uni(in), uni(out), uni(row0), uni(nrows), uni(smem), uni(tab), false);
 }
 
 template <int OP>
@@ -99,38 +126,38 @@ __device__ __forceinline__ void row_any(int w, const float *in, float *out, size
                                         const float *__restrict__ tab)
 {
     switch (w) {
-    case 8: row_w<8, OP>(in, out, row0, nrows, smem, tab); break;
-    case 16: row_w<16, OP>(in, out, row0, nrows, smem, tab); break;
-    case 32: row_w<32, OP>(in, out, row0, nrows, smem, tab); break;
-    case 64: row_w<64, OP>(in, out, row0, nrows, smem, tab); break;
-    case 128: row_w<128, OP>(in, out, row0, nrows, smem, tab); break;
-    case 256: row_w<256, OP>(in, out, row0, nrows, smem, tab); break;
-    case 512: row_w<512, OP>(in, out, row0, nrows, smem, tab); break;
-    default: row_w<1024, OP>(in, out, row0, nrows, smem, tab); break;
+    case 8: row_w<8, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 16: row_w<16, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 32: row_w<32, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 64: row_w<64, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 128: row_w<128, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 256: row_w<256, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    case 512: row_w<512, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
+    default: row_w<1024, OP>(to_g(in), to_g(out), row0, nrows, to_l(smem), to_g(tab)); break;
     }
 }
 
 template <int H>
-__device__ __noinline__ void col_h(const float *in, float *out, int w, size_t base, int c0, int ncols, float *smem,
-                                   const float *__restrict__ tab)
+__device__ __noinline__ void col_h(gcptr in, gptr out, int w, size_t base, int c0, int ncols, lptr smem,
+                                   gcptr __restrict__ tab)
 {
     static_assert(col_unit_lds<H, TH>() <= BDX_BATCH_LDS && col_unit_cols<H, TH>() == BDX_COL_STRIP(H),
                   "column unit geometry");
-    col_unit<H, BDX_MODE, TH>(in, out, w, base, c0, ncols, smem, tab, false);
+    col_unit<H, BDX_MODE, TH>(uni(in), uni(out), uni(w), uni(base), uni(c0), uni(ncols), uni(smem), uni(tab), false);
 }
 
 __device__ __forceinline__ void col_any(int h, int w, const float *in, float *out, size_t base, int c0, int ncols,
                                         float *smem, const float *__restrict__ tab)
 {
     switch (h) {
-    case 8: col_h<8>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 16: col_h<16>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 32: col_h<32>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 64: col_h<64>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 128: col_h<128>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 256: col_h<256>(in, out, w, base, c0, ncols, smem, tab); break;
-    case 512: col_h<512>(in, out, w, base, c0, ncols, smem, tab); break;
-    default: col_h<1024>(in, out, w, base, c0, ncols, smem, tab); break;
+    case 8: col_h<8>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 16: col_h<16>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 32: col_h<32>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 64: col_h<64>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 128: col_h<128>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 256: col_h<256>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    case 512: col_h<512>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
+    default: col_h<1024>(to_g(in), to_g(out), w, base, c0, ncols, to_l(smem), to_g(tab)); break;
     }
 }
 

@@ -133,7 +133,12 @@ template <int H, int THREADS> constexpr int col_unit_lds()
 }
 
 // Columns [c0, c0 + ncols) of the h x w array at `base` (in and out may alias),
-// ncols <= col_unit_cols; lanes past the strip compute on zeros and store nothing.
+// 1 <= ncols <= col_unit_cols.  The lines of lanes past a partial strip repeat its last
+// column: they read it, compute the same bits and store them to the same addresses (every
+// load of a block precedes its exchange barriers, every store follows them, so the
+// in-place repeat reads only input).  No access is predicated, so a strip width known only
+// at run time costs nothing (per-lane predicates on every access had taken the length-1024
+// unit from 196 to 248 VGPRs as a function of the persistent batch kernel).
 // MODE 0 forward * alpha, 1 alpha * inverse, 2 forward * alpha^2 then inverse.
 template <int H, int MODE, int THREADS, bool NT = false>
 __device__ __forceinline__ void col_unit(const float *in, float *out, int w, size_t base, int c0, int ncols,
@@ -153,12 +158,12 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int t = M == 1 ? __builtin_amdgcn_readfirstlane(wave) : wave * M + lane / L;
         const int line = lane % L;
-        const bool live = line < ncols;
+        const int col = line < ncols ? line : ncols - 1;
         // Row r of the strip is at the uniform strip base + r w floats.  The lane keeps one
         // byte offset per row family (lane part of the row, plus its column); each access
         // adds a uniform row offset in scalar registers (an array is at most 4 MiB, so the
         // offsets fit 32 bits).
-        const unsigned w4 = 4u * (unsigned)w, l4 = 4u * (unsigned)line;
+        const unsigned w4 = 4u * (unsigned)w, l4 = 4u * (unsigned)col;
         const int row_p = 4 * t, row_r = 4 * (T - 1 - t), row_c = t, row_d = T - 1 - t;
         // Low slots of the lead member (several members per wave, t = 0): lane part N1/2.  The
         // DL part can be negative, so it is kept biased by BL rows (and its uniform rows
@@ -182,11 +187,10 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         const char *src = reinterpret_cast<const char *>(in + base + c0);
         char *dst = reinterpret_cast<char *>(out + base + c0);
         const auto ld = [&](int fam, int urow) {
-            return live ? ld1<NT>(reinterpret_cast<const float *>(src + urow_of(fam, urow) * w4 + off(fam))) : 0.0f;
+            return ld1<NT>(reinterpret_cast<const float *>(src + urow_of(fam, urow) * w4 + off(fam)));
         };
         const auto st = [&](int fam, int urow, float v) {
-            if (live)
-                st1<NT>(reinterpret_cast<float *>(dst + urow_of(fam, urow) * w4 + off(fam)), v);
+            st1<NT>(reinterpret_cast<float *>(dst + urow_of(fam, urow) * w4 + off(fam)), v);
         };
         const float *al = M == 1 ? alpha_of<H>(tab, MODE == 2 ? 2 : 1) : lt + RowTab<H>::AL;
         // one pointer per row family, so that each factor is that pointer plus an immediate
