@@ -169,6 +169,7 @@ __device__ __forceinline__ void col_unit(const float *in, float *out, int w, siz
         // DL part can be negative, so it is kept biased by BL rows (and its uniform rows
         // lowered by as many): every offset stays a non-negative 32-bit value.
         constexpr int BL = C::N1 / 2;
+        static_assert(min_dl_urow<H, T>() >= BL, "lowered DL rows must stay non-negative");
         const int row_cl = M > 1 && t == 0 ? C::N1 / 2 : t, row_dl = T - 1 - row_cl + BL;
         const unsigned off_p = (unsigned)row_p * w4 + l4, off_r = (unsigned)row_r * w4 + l4;
         const unsigned off_c = (unsigned)row_c * w4 + l4, off_d = (unsigned)row_d * w4 + l4;

@@ -138,6 +138,16 @@ template <int n, int T> __device__ __forceinline__ constexpr int out_urow(int r,
          : which == 2 ? N - T + 1 - T * r - N1 * k2
                       : N + T * r + N1 * k2;
 }
+// Smallest uniform row of the DL family (outputs 1 and 2 of the low slots r = 0, k2 < T/2):
+// the column unit keeps that family biased by N1/2 rows, which this must not undercut.
+template <int n, int T> __device__ __forceinline__ constexpr int min_dl_urow()
+{
+    int m = n;
+    for (int k2 = 0; k2 < T / 2; k2++)
+        for (int w = 1; w <= 2; w++)
+            m = out_urow<n, T>(0, k2, w) < m ? out_urow<n, T>(0, k2, w) : m;
+    return m;
+}
 // Absolute output row (k2, which) of the self-paired group g = 0 (family C, lane part 0).
 template <int n, int T> __device__ __forceinline__ constexpr int out_row_g0(int k2, int which)
 {
