@@ -11,8 +11,9 @@ namespace bdx {
 enum Op { FA = 1, FA2 = 2, IA = 3, I0 = 4 };
 
 constexpr int kThreads = 256;
-// Accelerator complex dies (XCDs) of an MI355X; each has its own L2.
-constexpr unsigned kXcds = 8;
+// XCD count of the current device, read once per device (the two-pass kernels order their
+// workgroups by XCD; each XCD has its own L2).
+unsigned xcd_count();
 // Floats of LDS for one workgroup of the whole-array kernel (144 KiB of the 160 KiB).
 constexpr int kSmallLds = BDX_SMALL_LDS_FLOATS;
 // Floats of LDS per workgroup of the row / column pass kernels (two workgroups per CU).

@@ -35,10 +35,21 @@ template <int H, int MODE> constexpr int col_threads()
 // registers within three waves per SIMD (168 VGPRs).
 template <int W> constexpr int rows_min_blocks() { return W >= 512 ? 3 : 1; }
 
+// Block order that gives each XCD one contiguous range of the grid.  Workgroups are
+// dispatched to the `xcds` XCDs of the device round-robin (workgroup b on XCD b % xcds),
+// so block b takes position (b % xcds) (nb / xcds) + b / xcds: a permutation of [0, nb)
+// when nb is a multiple of xcds, else the plain order.  `xcds` is the device's XCD count
+// (8 on an MI355X in SPX mode; 1 in CPX mode, where the order is the plain one).
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned nb, unsigned xcds)
+{
+    return xcds > 1 && nb % xcds == 0 ? (b % xcds) * (nb / xcds) + b / xcds : b;
+}
+
 template <int W, int OP, bool G, bool NT>
 __global__ __launch_bounds__(kThreads, rows_min_blocks<W>()) void k_rows(const float *in, float *out,
                                                                           const size_t *__restrict__ offs, int h,
-                                                                          const float *__restrict__ tab)
+                                                                          const float *__restrict__ tab,
+                                                                          unsigned xcds)
 {
     __shared__ float smem[row_unit_lds<W, OP, kThreads>()];
     constexpr int R = row_unit_rows<W, kThreads>();
@@ -46,8 +57,7 @@ __global__ __launch_bounds__(kThreads, rows_min_blocks<W>()) void k_rows(const f
     // i.e. the same arrays its column strips take next.  2-3% faster on 64 x 128x1024 and
     // 64 x 512x128 (cache-resident calls), and the batch-64 four-shape step runs 4% faster,
     // although the 1024^2 column pass timed alone after it is 5% slower.
-    const unsigned nb = gridDim.x;
-    const unsigned bid = nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+    const unsigned bid = xcd_order(blockIdx.x, gridDim.x, xcds);
     const size_t row0 = (size_t)bid * R;
     if constexpr (G) {
         const size_t a = row0 / (unsigned)h, base = offs[a] + (row0 - a * h) * W;
@@ -60,17 +70,14 @@ __global__ __launch_bounds__(kThreads, rows_min_blocks<W>()) void k_rows(const f
 // Strips of TH / 64 * (64 / M) columns per workgroup of TH lanes.
 template <int H, int MODE, bool G, bool NT, int TH>
 __device__ __forceinline__ void cols_body(const float *in, float *out, int w, const size_t *__restrict__ offs,
-                                          const float *__restrict__ tab)
+                                          const float *__restrict__ tab, unsigned xcds)
 {
     constexpr int L = col_unit_cols<H, TH>();
     __shared__ float smem[col_unit_lds<H, TH>()];
     const int strips = w / L;
-    // Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD b % kXcds);
-    // the strips are renumbered so that each XCD takes one contiguous range of them
-    // (measured: 1024^2 transforms 5% faster than with consecutive strips on alternating
-    // XCDs).  A permutation of [0, gridDim.x) when the grid is a multiple of kXcds.
-    const unsigned nb = gridDim.x;
-    const unsigned bid = nb % kXcds == 0 ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+    // Each XCD takes one contiguous range of the strips (measured: 1024^2 transforms 5%
+    // faster than with consecutive strips on alternating XCDs).
+    const unsigned bid = xcd_order(blockIdx.x, gridDim.x, xcds);
     const size_t a = bid / strips;
     const int c0 = (int)(bid - a * strips) * L;
     col_unit<H, MODE, TH, NT>(in, out, w, G ? offs[a] : a * (size_t)H * w, c0, L, smem, tab, false);
@@ -79,9 +86,9 @@ __device__ __forceinline__ void cols_body(const float *in, float *out, int w, co
 template <int H, int MODE, bool G, bool NT>
 __global__ __launch_bounds__((col_threads<H, MODE>())) void k_cols(const float *in, float *out, int w,
                                                                   const size_t *__restrict__ offs,
-                                                                  const float *__restrict__ tab)
+                                                                  const float *__restrict__ tab, unsigned xcds)
 {
-    cols_body<H, MODE, G, NT, col_threads<H, MODE>()>(in, out, w, offs, tab);
+    cols_body<H, MODE, G, NT, col_threads<H, MODE>()>(in, out, w, offs, tab, xcds);
 }
 
 // The same column pass with narrower strips (TH = col_threads / 2 or / 4: 32 or 16
@@ -92,13 +99,14 @@ __global__ __launch_bounds__((col_threads<H, MODE>())) void k_cols(const float *
 template <int H, int MODE, bool G, int TH>
 __global__ __launch_bounds__(TH) void k_cols_narrow(const float *in, float *out, int w,
                                                      const size_t *__restrict__ offs,
-                                                     const float *__restrict__ tab)
+                                                     const float *__restrict__ tab, unsigned xcds)
 {
-    cols_body<H, MODE, G, false, TH>(in, out, w, offs, tab);
+    cols_body<H, MODE, G, false, TH>(in, out, w, offs, tab, xcds);
 }
 
 // Workgroups below which a column pass takes narrower strips: two per CU.
 constexpr size_t kColsMinGrid = 512;
+
 
 template <int W, bool G, bool NT>
 int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s, const size_t *offs,
@@ -109,10 +117,10 @@ int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab,
         return -1;
     const dim3 grid((unsigned)(nrows / R)), block(kThreads);
     switch (op) {
-    case FA: hipLaunchKernelGGL((k_rows<W, FA, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case FA2: hipLaunchKernelGGL((k_rows<W, FA2, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case IA: hipLaunchKernelGGL((k_rows<W, IA, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
-    case I0: hipLaunchKernelGGL((k_rows<W, I0, G, NT>), grid, block, 0, s, in, out, offs, h, tab); break;
+    case FA: hipLaunchKernelGGL((k_rows<W, FA, G, NT>), grid, block, 0, s, in, out, offs, h, tab, xcd_count()); break;
+    case FA2: hipLaunchKernelGGL((k_rows<W, FA2, G, NT>), grid, block, 0, s, in, out, offs, h, tab, xcd_count()); break;
+    case IA: hipLaunchKernelGGL((k_rows<W, IA, G, NT>), grid, block, 0, s, in, out, offs, h, tab, xcd_count()); break;
+    case I0: hipLaunchKernelGGL((k_rows<W, I0, G, NT>), grid, block, 0, s, in, out, offs, h, tab, xcd_count()); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -136,9 +144,9 @@ int cols_narrow(int w, const float *in, float *out, size_t batch, int mode, cons
     constexpr int L = col_unit_cols<H, TH>();
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_cols_narrow<H, 0, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 1: hipLaunchKernelGGL((k_cols_narrow<H, 1, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
-    case 2: hipLaunchKernelGGL((k_cols_narrow<H, 2, G, TH>), grid, block, 0, s, in, out, w, offs, tab); break;
+    case 0: hipLaunchKernelGGL((k_cols_narrow<H, 0, G, TH>), grid, block, 0, s, in, out, w, offs, tab, xcd_count()); break;
+    case 1: hipLaunchKernelGGL((k_cols_narrow<H, 1, G, TH>), grid, block, 0, s, in, out, w, offs, tab, xcd_count()); break;
+    case 2: hipLaunchKernelGGL((k_cols_narrow<H, 2, G, TH>), grid, block, 0, s, in, out, w, offs, tab, xcd_count()); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -166,7 +174,7 @@ int cols_mode(int w, const float *in, float *out, size_t batch, const float *tab
         }
     }
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
-    hipLaunchKernelGGL((k_cols<H, MODE, G, NT>), grid, block, 0, s, in, out, w, offs, tab);
+    hipLaunchKernelGGL((k_cols<H, MODE, G, NT>), grid, block, 0, s, in, out, w, offs, tab, xcd_count());
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -194,6 +202,24 @@ int cols_h(int w, const float *in, float *out, size_t batch, int mode, const flo
 }
 
 }  // namespace
+
+// XCD count of the current device (hipDeviceAttributeNumberOfXccs), read once per device;
+// 1 when it cannot be read, which gives the plain block order.
+unsigned xcd_count()
+{
+    constexpr int kMaxDevices = 64;
+    static int count[kMaxDevices];   // 0: not read yet (racing first reads store the same value)
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return 1;
+    int c = __atomic_load_n(&count[dev], __ATOMIC_RELAXED);
+    if (c == 0) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || c < 1)
+            c = 1;
+        __atomic_store_n(&count[dev], c, __ATOMIC_RELAXED);
+    }
+    return (unsigned)c;
+}
 
 int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
                 const size_t *offs, int h, bool nt)

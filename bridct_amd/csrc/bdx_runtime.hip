@@ -174,6 +174,7 @@ int bdx_stream_create(void **stream)
     if (acquired(!injected_fault() && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess))
         return -1;
     *stream = (void *)s;
+    (void)bdx::xcd_count();   // read here, before any launch a caller may capture into a graph
     return 0;
 }
 
