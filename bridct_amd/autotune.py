@@ -2,7 +2,9 @@
 
 For every supported shape it times the candidates on the GPU -- the FP32 butterfly route
 with several whole-array packings (arrays per workgroup, for shapes whose array fits one
-workgroup's LDS) and, for shapes with both axes >= 64, the split-MFMA dense routes with
+workgroup's LDS), for those shapes also the row and column pass kernels instead (pack -1,
+where those kernels take the shape), and, for shapes with both axes >= 64, the
+split-MFMA dense routes with
 FP16 and BF16 operands -- on a synthetic batch of uniform[-1, 1) arrays, forward and
 inverse, with HIP events.  Every candidate is gated on error first: its max relative error
 against the FP64 transform (on two arrays of the batch) must stay within twice the
@@ -32,6 +34,7 @@ from . import capi
 LENGTHS = (8, 16, 32, 64, 128, 256, 512, 1024)
 SMALL_LDS_FLOATS = 36864      # whole-array route: h (w + 1) floats of LDS at most
 PACKS = (0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
+PACK_TWO_PASS = -1            # the row and column pass kernels instead of the whole-array one
 GATE_ARRAYS = 4               # arrays whose error gates a route
 ROUTE_NAMES = {capi.ROUTE_BUTTERFLY: 'butterfly', capi.ROUTE_MFMA_F16: 'mfma_f16',
                capi.ROUTE_MFMA_BF16: 'mfma_bf16'}
@@ -75,11 +78,20 @@ def dct_fp64(n):
     return np.sqrt(np.where(k == 0, 1.0, 2.0) / n) * np.cos(np.pi * ((k * (2 * j + 1)) % (4 * n)) / (2.0 * n))
 
 
+def two_pass_ok(h, w):
+    """Whether the row and column pass kernels take an h x w shape (no partial bands: the
+    width-64 row kernel runs bands of 256 rows, the height-64 column kernel strips of 256
+    columns)."""
+    return ((w >= 128 or (w == 64 and h % 256 == 0)) and (h >= 128 or (h == 64 and w % 256 == 0)))
+
+
 def candidates(h, w, batch):
     out = []
     if h * (w + 1) <= SMALL_LDS_FLOATS:
         per_max = SMALL_LDS_FLOATS // (h * (w + 1))
         out += [(capi.ROUTE_BUTTERFLY, p) for p in PACKS if p <= min(per_max, batch)]
+        if two_pass_ok(h, w):
+            out.append((capi.ROUTE_BUTTERFLY, PACK_TWO_PASS))
     else:
         out.append((capi.ROUTE_BUTTERFLY, 0))
     if h >= 64 and w >= 64:
@@ -108,6 +120,7 @@ def tune_shape(ev, h, w, pixels):
             ms = {}
             err = 0.0
             for mode in (capi.FORWARD, capi.INVERSE):
+                assert plan.apply_device(mode, d_in.ptr, d_out.ptr, batch) == 0, (h, w, route, pack)
                 ms[mode] = ev.time_ms(plan.stream, lambda: plan.apply_device(mode, d_in.ptr, d_out.ptr, batch), 3)
                 assert plan.synchronize() == 0
                 y = np.empty_like(x[:GATE_ARRAYS])

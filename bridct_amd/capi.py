@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, 'libbridct_amd.so')
 
 FORWARD, INVERSE, ROUNDTRIP, FORWARD_THEN_INVERSE = 0, 1, 2, 3
 ROUTE_BUTTERFLY, ROUTE_MFMA_F16, ROUTE_MFMA_BF16 = 0, 1, 2
+PACK_TWO_PASS = -1   # row and column pass kernels instead of the whole-array kernel
 BATCH_AUTO, BATCH_PERSISTENT, BATCH_GROUPED = 0, 1, 2
 OK, INVALID_ARGUMENT, WORKSPACE_TOO_SMALL, BAD_ALIGNMENT, OVERLAPPING_BUFFERS, DEVICE_ERROR = (
     0, -1, -2, -3, -4, -5)

@@ -297,6 +297,8 @@ static int build_groups(bridct_batch *b, const int *heights, const int *widths, 
                 route = BRIDCT_ROUTE_BUTTERFLY, pack = 0;
             if (route != BRIDCT_ROUTE_BUTTERFLY && (g->h < 64 || g->w < 64))
                 route = BRIDCT_ROUTE_BUTTERFLY;   /* the dense route needs both axes >= 64 */
+            if (pack == BDX_PACK_TWO_PASS && !BDX_TWO_PASS_OK(g->h, g->w))
+                pack = 0;                         /* a shape the two-pass kernels do not take */
             g->route = route;
             g->pack = pack;
         }
@@ -554,7 +556,7 @@ static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d
 {
     unsigned nsmall = 0;
     for (unsigned i = 0; i < b->ngroups; i++)
-        nsmall += BDX_SMALL_ROUTE(b->groups[i].h, b->groups[i].w);
+        nsmall += BDX_WHOLE_ARRAY(b->groups[i].h, b->groups[i].w, b->groups[i].pack);
     const int fork = nsmall > 0 && nsmall < b->ngroups;
     if (fork && (bdx_event_record(b->ev_fork, b->stream) || bdx_stream_wait_event(b->aux_stream, b->ev_fork)))
         return -1;
@@ -565,7 +567,7 @@ static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d
             rc = launch_dense(b, g, mode, d_in, d_out);
             continue;
         }
-        void *s = fork && BDX_SMALL_ROUTE(g->h, g->w) ? b->aux_stream : b->stream;
+        void *s = fork && BDX_WHOLE_ARRAY(g->h, g->w, g->pack) ? b->aux_stream : b->stream;
         rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
                                   b->d_tables, s);
     }

@@ -21,6 +21,7 @@ int small_dispatch(int h, int w, const float *in, float *out, size_t batch, int 
                    const float *tab, hipStream_t s, const size_t *offs = nullptr)
 {
     using namespace bdx;
+    pack = pack < 0 ? 0 : pack;   // the two-pass value on a shape that only fits this kernel
     switch (h) {
     case 8: return launch_small_h8(w, in, out, batch, pack, mode, tab, s, offs);
     case 16: return launch_small_h16(w, in, out, batch, pack, mode, tab, s, offs);
@@ -281,7 +282,7 @@ int bdx_transform_pass(const float *d_in, float *d_out, int h, int w, size_t bat
     if ((mode != BDX_FORWARD && mode != BDX_INVERSE) || (pass != 0 && pass != 1))
         return -1;
     hipStream_t s = (hipStream_t)stream;
-    if (bdx::small_route(h, w))
+    if (BDX_WHOLE_ARRAY(h, w, pack))
         return pass == 0 ? small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s) : 0;
     const bool nt = 2 * batch * (size_t)h * w * sizeof(float) > BDX_STREAM_MIN_BYTES;
     if (pass == 0)
@@ -297,7 +298,7 @@ static int transform(const float *d_in, float *d_out, int h, int w, size_t batch
         return 0;
     if (mode < BDX_FORWARD || mode > BDX_ROUNDTRIP)
         return -1;
-    if (bdx::small_route(h, w))
+    if (BDX_WHOLE_ARRAY(h, w, pack))
         return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s, offs);
     const size_t nrows = batch * (size_t)h;
     const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;

@@ -51,6 +51,16 @@ extern "C" {
  * is transformed by one workgroup in LDS. */
 #define BDX_SMALL_LDS_FLOATS 36864
 #define BDX_SMALL_ROUTE(h, w) ((long)(h) * ((w) + 1) <= BDX_SMALL_LDS_FLOATS)
+/* Shapes of that route that the row and column pass kernels also take: the row kernel of
+ * width 64 runs bands of 256 rows, the column kernel of height 64 strips of 256 columns;
+ * from 128 up both take any power-of-two extent of 64 or more. */
+#define BDX_TWO_PASS_OK(h, w) \
+    (((w) >= 128 || ((w) == 64 && (h) % 256 == 0)) && ((h) >= 128 || ((h) == 64 && (w) % 256 == 0)))
+/* Pack value that sends such a shape to the two-pass kernels (BRIDCT_PACK_TWO_PASS). */
+#define BDX_PACK_TWO_PASS (-1)
+/* Whether an h x w shape with packing `pack` runs on the whole-array kernel. */
+#define BDX_WHOLE_ARRAY(h, w, pack) \
+    (BDX_SMALL_ROUTE(h, w) && !((pack) == BDX_PACK_TWO_PASS && BDX_TWO_PASS_OK(h, w)))
 
 /* Batches whose arrays all have both axes at most BDX_TINY_MAX have no dependent items: they
  * run the tiny kernel instead of the persistent queue, one workgroup per item, about

@@ -206,7 +206,8 @@ bridct_plan *bridct_plan_create(int height, int width)
      * route this plan cannot take (e.g. a table written for other shapes) is ignored */
     int route = BRIDCT_ROUTE_BUTTERFLY, pack = 0;
     if (bridct_schedule_lookup(height, width, &route, &pack) == 0) {
-        plan->pack = pack;
+        if (bridct_plan_set_pack(plan, pack) != BRIDCT_OK)
+            plan->pack = 0;
         if (route != BRIDCT_ROUTE_BUTTERFLY && bridct_plan_set_route(plan, route) != BRIDCT_OK)
             plan->route = BRIDCT_ROUTE_BUTTERFLY;
     }
@@ -296,7 +297,8 @@ int bridct_plan_get_route(const bridct_plan *plan)
 
 int bridct_plan_set_pack(bridct_plan *plan, int pack)
 {
-    if (!plan || pack < 0 || pack > 4096)
+    if (!plan || pack < BRIDCT_PACK_TWO_PASS || pack > 4096 ||
+        (pack == BRIDCT_PACK_TWO_PASS && !BDX_TWO_PASS_OK(plan->height, plan->width)))
         return BRIDCT_INVALID_ARGUMENT;
     plan->pack = pack;
     return BRIDCT_OK;

@@ -8,7 +8,8 @@
  *     <height> <width> <route> <pack>
  *
  * route: butterfly | mfma_f16 | mfma_bf16; pack: arrays per workgroup of the whole-array
- * kernel (0 = automatic).  '#' starts a comment.  The file is $BRIDCT_SCHEDULE when that
+ * kernel (0 = automatic), or -1 (BRIDCT_PACK_TWO_PASS): the row and column pass kernels
+ * instead of the whole-array kernel.  '#' starts a comment.  The file is $BRIDCT_SCHEDULE when that
  * variable is set (empty or "none": no table), else schedule_gfx950.txt in the directory
  * of this library.  A shape missing from the table, or a missing table, keeps the
  * defaults (butterfly, automatic packing).
@@ -52,7 +53,7 @@ int bdx_schedule_lookup_file(const char *path, int h, int w, int *route, int *pa
         if (lh != h || lw != w)
             continue;
         const int r = route_id(name);
-        if (r < 0 || lp < 0 || lp > 4096)
+        if (r < 0 || lp < BRIDCT_PACK_TWO_PASS || lp > 4096)
             continue;
         *route = r;
         *pack = lp;

@@ -55,7 +55,12 @@ BRIDCT_API int bridct_plan_get_route(const bridct_plan *plan);
 
 /* Arrays per workgroup of the whole-array kernel (shapes whose array fits one
  * workgroup's LDS): 0 = automatic, else 1..4096 (clamped to what fits, and lowered for
- * batches too small to give about 512 workgroups at that packing). */
+ * batches too small to give about 512 workgroups at that packing).  BRIDCT_PACK_TWO_PASS
+ * runs such a shape through the row and column pass kernels instead, as larger shapes
+ * are; it is accepted for the shapes those kernels take without partial bands: width at
+ * least 128 and height at least 64, or width 64 and height 256 or 512 (and a height of 64
+ * needs a width of 256 or 512).  The autotuner picks it where it measured faster. */
+#define BRIDCT_PACK_TWO_PASS (-1)
 BRIDCT_API int bridct_plan_set_pack(bridct_plan *plan, int pack);
 BRIDCT_API int bridct_plan_get_pack(const bridct_plan *plan);
 

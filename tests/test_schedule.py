@@ -7,7 +7,8 @@ import os
 import numpy as np
 import pytest
 
-from bridct_amd import capi
+import support as S
+from bridct_amd import autotune, capi
 
 
 def lookup(path, h, w):
@@ -31,19 +32,22 @@ def test_parse_table(tmp_path):
                  '64 64 butterfly 4\n'          # the last matching line wins
                  '32 32 warp_drive 1\n'         # unknown route: ignored
                  '16 16 butterfly -3\n'         # bad pack: ignored
+                 '128 128 butterfly -1\n'       # the two-pass kernels
                  'garbage\n')
     assert lookup(t, 8, 8) == (0, capi.ROUTE_BUTTERFLY, 64)
     assert lookup(t, 256, 256) == (0, capi.ROUTE_MFMA_F16, 0)
     assert lookup(t, 64, 64) == (0, capi.ROUTE_BUTTERFLY, 4)
     assert lookup(t, 32, 32)[0] == -1
     assert lookup(t, 16, 16)[0] == -1
+    assert lookup(t, 128, 128) == (0, capi.ROUTE_BUTTERFLY, capi.PACK_TWO_PASS)
     assert lookup(t, 1024, 8)[0] == -1
     assert lookup(tmp_path / 'missing.txt', 8, 8)[0] == -1
 
 
 def test_committed_table_is_complete():
     """The committed table (written by the autotuner on an MI355X) covers all 64 shapes with
-    a valid route; dense routes only where both axes are at least 64."""
+    a valid route; dense routes only where both axes are at least 64; the two-pass choice
+    only for whole-array shapes that the row and column kernels take."""
     path = os.path.join(os.path.dirname(capi.LIB_PATH), 'schedule_gfx950.txt')
     if not os.path.exists(path):
         pytest.skip('no schedule table committed yet')
@@ -52,25 +56,40 @@ def test_committed_table_is_complete():
             rc, route, pack = lookup(path, h, w)
             assert rc == 0, (h, w)
             assert route == capi.ROUTE_BUTTERFLY or (h >= 64 and w >= 64)
-            assert 0 <= pack <= 4096
+            assert 0 <= pack <= 4096 or (pack == capi.PACK_TWO_PASS and autotune.two_pass_ok(h, w)
+                                         and h * (w + 1) <= autotune.SMALL_LDS_FLOATS), (h, w, pack)
+
+
+def test_two_pass_eligibility():
+    """The shapes the two-pass choice is offered for: every whole-array shape with both axes
+    at least 64 except 64 x 64, 64 x 128 and 128 x 64, whose bands or strips would be
+    partial (width-64 row bands hold 256 rows, height-64 column strips 256 columns)."""
+    small = [(h, w) for h in (8, 16, 32, 64, 128, 256, 512, 1024) for w in (8, 16, 32, 64, 128, 256, 512, 1024)
+             if h * (w + 1) <= autotune.SMALL_LDS_FLOATS]
+    eligible = sorted(s for s in small if autotune.two_pass_ok(*s))
+    assert eligible == [(64, 256), (64, 512), (128, 128), (128, 256), (256, 64), (256, 128), (512, 64)]
 
 
 @pytest.mark.gpu
 def test_table_sets_plan_route_and_pack(tmp_path, monkeypatch):
     t = tmp_path / 'sched.txt'
-    t.write_text('256 256 mfma_f16 0\n16 16 butterfly 8\n32 32 mfma_f16 0\n')
+    t.write_text('256 256 mfma_f16 0\n16 16 butterfly 8\n32 32 mfma_f16 0\n128 128 butterfly -1\n'
+                 '64 64 butterfly -1\n')
     monkeypatch.setenv('BRIDCT_SCHEDULE', str(t))
     p = capi.Plan(256, 256)
     q = capi.Plan(16, 16)
     r = capi.Plan(32, 32)      # dense route impossible below 64: stays butterfly
+    u = capi.Plan(128, 128)
+    v = capi.Plan(64, 64)      # two-pass kernels do not take 64 x 64: automatic packing
     try:
         assert p.route == capi.ROUTE_MFMA_F16 and p.pack == 0
         assert q.route == capi.ROUTE_BUTTERFLY and q.pack == 8
         assert r.route == capi.ROUTE_BUTTERFLY
+        assert u.route == capi.ROUTE_BUTTERFLY and u.pack == capi.PACK_TWO_PASS
+        assert v.route == capi.ROUTE_BUTTERFLY and v.pack == 0
     finally:
-        p.close()
-        q.close()
-        r.close()
+        for x in (p, q, r, u, v):
+            x.close()
     monkeypatch.setenv('BRIDCT_SCHEDULE', 'none')
     p = capi.Plan(256, 256)
     try:
@@ -93,6 +112,38 @@ def test_packing_does_not_change_results(h, w):
             outs.append(y)
         for y in outs[1:]:
             assert np.array_equal(y, outs[0])
-        assert p.set_pack(-1) == capi.INVALID_ARGUMENT and p.set_pack(4097) == capi.INVALID_ARGUMENT
+        assert p.set_pack(-2) == capi.INVALID_ARGUMENT and p.set_pack(4097) == capi.INVALID_ARGUMENT
+        assert (p.set_pack(capi.PACK_TWO_PASS) == 0) == autotune.two_pass_ok(h, w)
     finally:
+        p.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('h,w', [(64, 256), (64, 512), (128, 128), (128, 256), (256, 64), (256, 128), (512, 64)])
+def test_two_pass_choice_on_whole_array_shapes(h, w):
+    """A whole-array shape sent to the row and column kernels (pack -1) meets the FP64
+    oracle in every mode, as a plain call and through the per-pass entry point (both passes
+    in order), and its pass-split call equals its plain call bitwise."""
+    print('two-pass %dx%d' % (h, w), flush=True)
+    x = np.random.default_rng(h * 3 + w).uniform(-1, 1, (5, h, w)).astype(np.float32)
+    p = capi.Plan(h, w)
+    a, b = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+    try:
+        assert p.set_pack(capi.PACK_TWO_PASS) == 0 and p.pack == capi.PACK_TWO_PASS
+        for mode in (capi.FORWARD, capi.INVERSE, capi.ROUNDTRIP):
+            y = np.empty_like(x)
+            assert p.apply_batch(mode, x, y) == 0
+            assert S.max_rel(y, S.oracle_2d(x, mode)) < 1e-6, mode
+        a.upload(x)
+        for mode in (capi.FORWARD, capi.INVERSE):
+            assert p.apply_device_pass(mode, 0, a.ptr, b.ptr, 5) == 0
+            assert p.apply_device_pass(mode, 1, b.ptr, b.ptr, 5) == 0
+            assert p.synchronize() == 0
+            y, z = np.empty_like(x), np.empty_like(x)
+            b.download(y)
+            assert p.apply_batch(mode, x, z) == 0
+            assert np.array_equal(y, z), mode
+    finally:
+        a.free()
+        b.free()
         p.close()
