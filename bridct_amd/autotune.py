@@ -10,7 +10,9 @@ inverse, with HIP events.  Every candidate is gated on error first: its max rela
 against the FP64 transform (on two arrays of the batch) must stay within twice the
 butterfly route's error on the same arrays (the butterfly route itself sits within twice
 the reference CPU's error, which the parity tests check).  The fastest candidate that
-passes is written to the table, one line per shape:
+passes is written to the table, one line per shape (the two-pass kernels only where they
+also beat automatic whole-array packing on a batch of 64 arrays by 25%, since a plan
+serves any batch and the two passes are two launches):
 
     <height> <width> <butterfly|mfma_f16|mfma_bf16> <pack>
 
@@ -35,6 +37,11 @@ LENGTHS = (8, 16, 32, 64, 128, 256, 512, 1024)
 SMALL_LDS_FLOATS = 36864      # whole-array route: h (w + 1) floats of LDS at most
 PACKS = (0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
 PACK_TWO_PASS = -1            # the row and column pass kernels instead of the whole-array one
+SMALL_BATCH = 64              # arrays of the second comparison a two-pass choice must also win
+# ... by this factor: the two passes are two launches per direction, a cost that isolated
+# timing does not charge (a batch-64 step of four shapes on four streams ran 1.2% slower
+# with 32 x 256 on two passes, although that shape alone ran 18% faster)
+TWO_PASS_MARGIN = 1.25
 GATE_ARRAYS = 4               # arrays whose error gates a route
 ROUTE_NAMES = {capi.ROUTE_BUTTERFLY: 'butterfly', capi.ROUTE_MFMA_F16: 'mfma_f16',
                capi.ROUTE_MFMA_BF16: 'mfma_bf16'}
@@ -79,10 +86,13 @@ def dct_fp64(n):
 
 
 def two_pass_ok(h, w):
-    """Whether the row and column pass kernels take an h x w shape (no partial bands: the
-    width-64 row kernel runs bands of 256 rows, the height-64 column kernel strips of 256
-    columns)."""
-    return ((w >= 128 or (w == 64 and h % 256 == 0)) and (h >= 128 or (h == 64 and w % 256 == 0)))
+    """Whether the row and column pass kernels take an h x w shape in whole row bands and
+    column strips (BDX_TWO_PASS_OK: bands of 256 / team rows, 256 for lines up to 64;
+    strips of 64 columns, 32 at height 1024, 256 for lines up to 64)."""
+    team = 1 if w <= 64 else 16 if w >= 1024 else 8 if w >= 256 else 4
+    rows = 256 if w <= 64 else 256 // team
+    cols = 32 if h >= 1024 else 64 if h >= 128 else 256
+    return h % rows == 0 and w % cols == 0
 
 
 def candidates(h, w, batch):
@@ -135,7 +145,33 @@ def tune_shape(ev, h, w, pixels):
     base = max(r['err'] for r in results if r['route'] == capi.ROUTE_BUTTERFLY and r['pack'] == 0)
     ok = [r for r in results if r['err'] <= 2 * base]
     best = max(ok, key=lambda r: r['gpx'])
-    return batch, best, results
+    small = None
+    if best['pack'] == PACK_TWO_PASS and batch > SMALL_BATCH:
+        # the two-pass kernels spread each array over many workgroups, which pays on large
+        # batches; a plan serves any batch, so the choice must also win on a small one
+        small = small_batch_rates(ev, h, w, d_in, d_out)
+        if small[PACK_TWO_PASS] < TWO_PASS_MARGIN * small[0]:
+            best = max((r for r in ok if r['pack'] != PACK_TWO_PASS), key=lambda r: r['gpx'])
+    return batch, best, results, small
+
+
+def small_batch_rates(ev, h, w, d_in, d_out):
+    """Forward+inverse Gpixel/s of automatic whole-array packing and of the two-pass
+    kernels on SMALL_BATCH arrays (the first arrays of the tuning batch)."""
+    plan = capi.Plan(h, w)
+    rates = {}
+    try:
+        for pack in (0, PACK_TWO_PASS):
+            assert plan.set_pack(pack) == 0
+            ms = 0.0
+            for mode in (capi.FORWARD, capi.INVERSE):
+                assert plan.apply_device(mode, d_in.ptr, d_out.ptr, SMALL_BATCH) == 0
+                ms += min(ev.time_ms(plan.stream, lambda: plan.apply_device(mode, d_in.ptr, d_out.ptr, SMALL_BATCH),
+                                     20) for _ in range(3))
+            rates[pack] = 2 * SMALL_BATCH * h * w / (ms * 1e-3) / 1e9
+    finally:
+        plan.close()
+    return rates
 
 
 def main(argv=None):
@@ -157,11 +193,15 @@ def main(argv=None):
                  args.pixels, datetime.datetime.utcnow().strftime('%Y-%m-%d %H:%M')),
              '# gate: max relative error vs FP64 <= 2 x the butterfly route (pack 0) on the same %d arrays,' % GATE_ARRAYS,
              '# scaled by 10^U[-3, 3] each;',
-             '# then the highest forward+inverse Gpixel/s.  Comment: every candidate as route/pack:Gpx/s:error.']
+             '# then the highest forward+inverse Gpixel/s; the two-pass kernels (pack -1) only where they',
+             '# also beat automatic whole-array packing on %d arrays by a factor %.2f.' % (SMALL_BATCH, TWO_PASS_MARGIN),
+             '# Comment: every candidate as route/pack:Gpx/s:error.']
     t0 = time.time()
     for h, w in shapes:
-        batch, best, results = tune_shape(ev, h, w, args.pixels)
+        batch, best, results, small = tune_shape(ev, h, w, args.pixels)
         cand = ' '.join('%s/%d:%.1f:%.1e' % (ROUTE_NAMES[r['route']], r['pack'], r['gpx'], r['err']) for r in results)
+        if small:
+            cand += '; on %d arrays: butterfly/0:%.1f butterfly/-1:%.1f' % (SMALL_BATCH, small[0], small[PACK_TWO_PASS])
         lines.append('%d %d %s %d  # batch %d; %s' % (h, w, ROUTE_NAMES[best['route']], best['pack'], batch, cand))
         print(lines[-1], flush=True)
     with open(args.out, 'w') as f:

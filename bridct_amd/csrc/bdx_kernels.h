@@ -55,6 +55,12 @@ int launch_rows(int w, const float *in, float *out, size_t nrows, int op, const 
 // 1 alpha*inverse, 2 forward*alpha^2 then inverse.
 int launch_cols(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
                 hipStream_t s, const size_t *offs = nullptr, bool nt = false);
+// The same for lines of 8, 16 and 32 (bdx_passes_lines.hip); launch_rows and launch_cols
+// hand those lengths on to them.
+int launch_rows_lines(int w, const float *in, float *out, size_t nrows, int op, const float *tab, hipStream_t s,
+                      const size_t *offs, int h, bool nt);
+int launch_cols_lines(int h, int w, const float *in, float *out, size_t batch, int mode, const float *tab,
+                      hipStream_t s, const size_t *offs, bool nt);
 
 // Persistent batch kernel, one per mode (bdx_batch.hip); flags: BDX_LAUNCH_*.
 int launch_batch_m0(int flags, const float *in, float *out, const bdx_run *runs, const bdx_item *items, unsigned nitems,

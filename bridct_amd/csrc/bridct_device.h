@@ -51,11 +51,13 @@ extern "C" {
  * is transformed by one workgroup in LDS. */
 #define BDX_SMALL_LDS_FLOATS 36864
 #define BDX_SMALL_ROUTE(h, w) ((long)(h) * ((w) + 1) <= BDX_SMALL_LDS_FLOATS)
-/* Shapes of that route that the row and column pass kernels also take: the row kernel of
- * width 64 runs bands of 256 rows, the column kernel of height 64 strips of 256 columns;
- * from 128 up both take any power-of-two extent of 64 or more. */
-#define BDX_TWO_PASS_OK(h, w) \
-    (((w) >= 128 || ((w) == 64 && (h) % 256 == 0)) && ((h) >= 128 || ((h) == 64 && (w) % 256 == 0)))
+/* Shapes of that route that the row and column pass kernels also take: whole bands and
+ * strips.  A row band is 256 / team rows (256 rows for lines up to 64, one lane per line);
+ * a column strip is 64 columns (32 at height 1024; 256 for lines up to 64).  The pass
+ * kernels check these sizes against their own at compile time. */
+#define BDX_PASS_ROWS(w) ((w) >= 128 ? 256 / BDX_TEAM(w) : 256)
+#define BDX_PASS_COLS(h) ((h) >= 1024 ? 32 : (h) >= 128 ? 64 : 256)
+#define BDX_TWO_PASS_OK(h, w) ((h) % BDX_PASS_ROWS(w) == 0 && (w) % BDX_PASS_COLS(h) == 0)
 /* Pack value that sends such a shape to the two-pass kernels (BRIDCT_PACK_TWO_PASS). */
 #define BDX_PACK_TWO_PASS (-1)
 /* Whether an h x w shape with packing `pack` runs on the whole-array kernel. */

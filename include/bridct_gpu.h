@@ -57,9 +57,10 @@ BRIDCT_API int bridct_plan_get_route(const bridct_plan *plan);
  * workgroup's LDS): 0 = automatic, else 1..4096 (clamped to what fits, and lowered for
  * batches too small to give about 512 workgroups at that packing).  BRIDCT_PACK_TWO_PASS
  * runs such a shape through the row and column pass kernels instead, as larger shapes
- * are; it is accepted for the shapes those kernels take without partial bands: width at
- * least 128 and height at least 64, or width 64 and height 256 or 512 (and a height of 64
- * needs a width of 256 or 512).  The autotuner picks it where it measured faster. */
+ * are; it is accepted for the shapes those kernels take in whole row bands (256 rows for
+ * widths up to 64, else 256 / team size: 64, 32, 32, 16 rows at widths 128..1024) and
+ * whole column strips (256 columns for heights up to 64, 64 above, 32 at height 1024).
+ * The autotuner picks it where it measured faster. */
 #define BRIDCT_PACK_TWO_PASS (-1)
 BRIDCT_API int bridct_plan_set_pack(bridct_plan *plan, int pack);
 BRIDCT_API int bridct_plan_get_pack(const bridct_plan *plan);

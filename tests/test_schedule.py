@@ -61,13 +61,17 @@ def test_committed_table_is_complete():
 
 
 def test_two_pass_eligibility():
-    """The shapes the two-pass choice is offered for: every whole-array shape with both axes
-    at least 64 except 64 x 64, 64 x 128 and 128 x 64, whose bands or strips would be
-    partial (width-64 row bands hold 256 rows, height-64 column strips 256 columns)."""
+    """The shapes the two-pass choice is offered for: the whole-array shapes whose row bands
+    and column strips are whole (for lines up to 64, bands of 256 rows and strips of 256
+    columns)."""
     small = [(h, w) for h in (8, 16, 32, 64, 128, 256, 512, 1024) for w in (8, 16, 32, 64, 128, 256, 512, 1024)
              if h * (w + 1) <= autotune.SMALL_LDS_FLOATS]
     eligible = sorted(s for s in small if autotune.two_pass_ok(*s))
-    assert eligible == [(64, 256), (64, 512), (128, 128), (128, 256), (256, 64), (256, 128), (512, 64)]
+    assert eligible == [(16, 1024), (32, 256), (32, 512), (32, 1024), (64, 256), (64, 512), (128, 128), (128, 256),
+                        (256, 64), (256, 128), (512, 64), (1024, 32)]
+    # every shape too large for the whole-array kernel is one the two-pass kernels take
+    assert all(autotune.two_pass_ok(h, w) for h in (8, 16, 32, 64, 128, 256, 512, 1024)
+               for w in (8, 16, 32, 64, 128, 256, 512, 1024) if h * (w + 1) > autotune.SMALL_LDS_FLOATS)
 
 
 @pytest.mark.gpu
@@ -119,7 +123,8 @@ def test_packing_does_not_change_results(h, w):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('h,w', [(64, 256), (64, 512), (128, 128), (128, 256), (256, 64), (256, 128), (512, 64)])
+@pytest.mark.parametrize('h,w', [(16, 1024), (32, 256), (32, 512), (32, 1024), (64, 256), (64, 512), (128, 128),
+                                 (128, 256), (256, 64), (256, 128), (512, 64), (1024, 32)])
 def test_two_pass_choice_on_whole_array_shapes(h, w):
     """A whole-array shape sent to the row and column kernels (pack -1) meets the FP64
     oracle in every mode, as a plain call and through the per-pass entry point (both passes
