@@ -87,8 +87,7 @@ def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError('libbridct_amd.so is not built; run __graft_entry__.build() '
-                              'or make -C bridct_amd')
+            raise ImportError('libbridct_amd.so is not built; run make -C bridct_amd')
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

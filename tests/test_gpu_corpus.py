@@ -5,7 +5,7 @@ unchanged, guard sentinels intact, input-relative Frobenius error < 2e-5.  Gate 
 "no CPU pass turns into a GPU failure" rule of the reference's numerical comparisons):
 every direction the reference CPU library passes (tests/golden/corpus_summary.json) must
 pass on the GPU; all core directions pass; the GPU's core maximum stays within twice the
-reference's.  Writes gpurun_out/corpus_gpu.json."""
+reference's.  Writes the per-direction results as JSON to the run's output folder."""
 import json
 import os
 

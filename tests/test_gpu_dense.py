@@ -1,8 +1,8 @@
 """GPU tests of the dense split-MFMA route (Y = C_h X C_w^T on the matrix cores from FP16
 or BF16 hi + lo operands, three products, FP32 accumulation): error against the FP64
 oracle for every shape with both axes >= 64, both directions, at scales 1, 1e-3 and 1e3,
-with the butterfly route's error on the same arrays (written to
-gpurun_out/dense_errors.json), exact power-of-two scale equivariance, and the route's API
+with the butterfly route's error on the same arrays (written as a JSON table to the run's
+output folder), exact power-of-two scale equivariance, and the route's API
 rules.  The reference has no dense route; its definition of the transform is
 docs/API.md:7-13 (the oracle's Q matrices)."""
 import json

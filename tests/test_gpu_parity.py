@@ -1,7 +1,7 @@
 """GPU parity: every one of the 64 shapes and 3 modes against the FP64 oracle and against
 the reference CPU library's output on the same seeded inputs.
 
-Bars (BASELINE north star):
+Bars:
   * max|y - ref| / max|ref| (vs FP64) at most 2x the reference CPU path's on the same input;
   * ||y_gpu - y_cpu|| / ||y_cpu|| <= 1e-6 against the reference CPU output;
   * the reference's own acceptance rule ||y - ref||_F / ||x||_F <= 2e-5
