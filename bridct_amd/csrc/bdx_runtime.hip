@@ -5,10 +5,13 @@
 // src/bridct.c:47-74):
 //   * h * (w + 1) floats fit 144 KiB of LDS: one whole-array kernel (bdx_small.hip);
 //   * otherwise: row pass (input -> output) then column pass in place on the output
-//     (bdx_passes.hip); the fused round trip adds a third, inverse row pass.
+//     (bdx_passes.hip); the fused round trip adds a third, inverse row pass.  Calls whose
+//     intermediate is at least twice the Infinity Cache run these passes per chunk of
+//     arrays (BDX_CHUNK_MIN_BYTES).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 
 #include "bdx_kernels.h"
@@ -300,14 +303,25 @@ static int transform(const float *d_in, float *d_out, int h, int w, size_t batch
         return -1;
     if (BDX_WHOLE_ARRAY(h, w, pack))
         return small_dispatch(h, w, d_in, d_out, batch, pack, mode, d_tables, s, offs);
-    const size_t nrows = batch * (size_t)h;
+    const size_t hw = (size_t)h * w;
     const int rop = mode == BDX_FORWARD ? bdx::FA : mode == BDX_INVERSE ? bdx::IA : bdx::FA2;
-    const bool nt = 2 * nrows * (size_t)w * sizeof(float) > BDX_STREAM_MIN_BYTES;
-    if (bdx::launch_rows(w, d_in, d_out, nrows, rop, d_tables, s, offs, h, nt) ||
-        bdx::launch_cols(h, w, d_out, d_out, batch, mode, d_tables, s, offs, nt))
-        return -1;
-    if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, d_out, d_out, nrows, bdx::I0, d_tables, s, offs, h, nt))
-        return -1;
+    // chunks of whole arrays when the intermediate is far larger than the Infinity Cache;
+    // the gather form advances its offset table, the contiguous form its array pointers
+    const size_t per = batch * hw * sizeof(float) >= BDX_CHUNK_MIN_BYTES
+                           ? std::max<size_t>(1, BDX_CHUNK_BYTES / (hw * sizeof(float)))
+                           : batch;
+    for (size_t a = 0; a < batch; a += per) {
+        const size_t n = std::min(per, batch - a), nrows = n * h;
+        const float *in = offs ? d_in : d_in + a * hw;
+        float *out = offs ? d_out : d_out + a * hw;
+        const size_t *o = offs ? offs + a : nullptr;
+        const bool nt = 2 * nrows * (size_t)w * sizeof(float) > BDX_STREAM_MIN_BYTES;
+        if (bdx::launch_rows(w, in, out, nrows, rop, d_tables, s, o, h, nt) ||
+            bdx::launch_cols(h, w, out, out, n, mode, d_tables, s, o, nt))
+            return -1;
+        if (mode == BDX_ROUNDTRIP && bdx::launch_rows(w, out, out, nrows, bdx::I0, d_tables, s, o, h, nt))
+            return -1;
+    }
     return 0;
 }
 

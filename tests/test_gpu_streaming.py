@@ -71,6 +71,75 @@ def test_1024_streaming_batch_against_oracle(batch, in_place):
         plan.close()
 
 
+CHUNK_MIN_BYTES = 512 << 20   # two-pass calls with this much intermediate run in chunks
+CHUNK_BYTES = 256 << 20
+
+
+def reference_in_pieces(plan, mode, x, piece=16):
+    """The transform of x (batch, h, w) by calls of `piece` arrays, each far below the chunk
+    threshold."""
+    y = np.empty_like(x)
+    for a in range(0, len(x), piece):
+        y[a:a + piece] = device_call(plan, mode, np.ascontiguousarray(x[a:a + piece]), len(x[a:a + piece]))
+    return y
+
+
+@pytest.mark.parametrize('in_place', [False, True])
+def test_1024_chunked_batch_bitwise(in_place):
+    """130 arrays of 1024^2 (520 MiB of intermediate): the call runs in chunks of 64, 64 and 2
+    arrays; every array bitwise equal to calls of 16 arrays, forward, inverse and round trip,
+    and the last array against the FP64 oracle."""
+    h = w = 1024
+    batch = 130
+    assert batch * h * w * 4 >= CHUNK_MIN_BYTES and batch % (CHUNK_BYTES // (h * w * 4))
+    x = np.random.default_rng(130).uniform(-1, 1, (batch, h, w)).astype(np.float32)
+    plan = capi.Plan(h, w)
+    try:
+        for mode in (0, 1, 2):
+            print('1024^2 x %d chunked, mode %d in_place %d' % (batch, mode, in_place), flush=True)
+            y = device_call(plan, mode, x, batch, in_place)
+            ref = reference_in_pieces(plan, mode, x)
+            bad = [i for i in range(batch) if not np.array_equal(y[i], ref[i])]
+            assert not bad, ('chunked call differs from 16-array calls', mode, bad[:8])
+            gate(y[-1], x[-1], mode)
+    finally:
+        plan.close()
+
+
+def test_grouped_chunked_gather_bitwise():
+    """A ragged batch with 140 arrays of 1024^2 (560 MiB of intermediate in that group: gather
+    calls in chunks of 64, 64 and 12) and a few 256x512 and 64x64 fillers, grouped strategy:
+    every array bitwise equal to a plan call on 16 arrays of its shape, both directions."""
+    shapes = [(1024, 1024)] * 70 + [(256, 512)] * 5 + [(1024, 1024)] * 70 + [(64, 64)] * 7
+    b = capi.Batch(shapes)
+    try:
+        assert b.set_strategy(capi.BATCH_GROUPED) == 0
+        x = np.random.default_rng(140).uniform(-1, 1, b.elements).astype(np.float32)
+        d_x, d_y = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
+        d_x.upload(x)
+        for mode in (capi.FORWARD, capi.INVERSE):
+            print('grouped 140 x 1024^2 chunked, mode %d' % mode, flush=True)
+            assert b.apply_device(mode, d_x.ptr, d_y.ptr) == 0 and b.synchronize() == 0
+            y = np.empty_like(x)
+            d_y.download(y)
+            for shape in sorted(set(shapes)):
+                idx = [i for i, s in enumerate(shapes) if s == shape]
+                h, w = shape
+                xs = np.stack([x[b.offsets[i]:b.offsets[i] + h * w].reshape(h, w) for i in idx])
+                plan = capi.Plan(h, w)
+                try:
+                    ref = reference_in_pieces(plan, mode, xs)
+                finally:
+                    plan.close()
+                for k, i in enumerate(idx):
+                    got = y[b.offsets[i]:b.offsets[i] + h * w].reshape(h, w)
+                    assert np.array_equal(got, ref[k]), ('grouped chunked result differs', shape, mode, i)
+        d_x.free()
+        d_y.free()
+    finally:
+        b.close()
+
+
 def test_ragged_10000_grouped_streaming_against_oracle():
     """The ragged mix of 10,000 arrays (seed 7, 2.7 GB per buffer): the automatic strategy
     (grouped, per-shape kernels in gather form, streaming hints) in one call; for every
