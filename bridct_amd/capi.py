@@ -73,6 +73,7 @@ SIGNATURES = {
     'bridct_codec8_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_codec8_stream': (ctypes.c_void_p, [ctypes.c_void_p]),
     'bridct_debug_alloc_faults': (ctypes.c_long, [ctypes.c_long]),
+    'bridct_debug_grid_limit': (ctypes.c_long, [ctypes.c_long]),
     'bridct_gpu_malloc': (ctypes.c_void_p, [ctypes.c_size_t]),
     'bridct_gpu_free': (ctypes.c_int, [ctypes.c_void_p]),
     'bridct_gpu_upload': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),

@@ -400,13 +400,22 @@ int BDX_CAT(launch_batch_m, BDX_MODE)(int flags, const float *in, float *out, co
                                       unsigned nunits, unsigned qbase, unsigned *ctr, const float *tab,
                                       unsigned blocks, unsigned spin_limit, hipStream_t s)
 {
-    if ((flags & BDX_LAUNCH_TINY) && (flags & BDX_LAUNCH_STREAM))
-        hipLaunchKernelGGL(k_batch_tiny<true>, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
-    else if (flags & BDX_LAUNCH_TINY)
-        hipLaunchKernelGGL(k_batch_tiny<false>, dim3(nitems), dim3(TH), 0, s, in, out, runs, items, tab);
-    else
-        hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, units, nunits, qbase, ctr,
-                           tab, spin_limit);
+    if (flags & BDX_LAUNCH_TINY) {
+        // one workgroup per item; item lists longer than a launch allows run in several
+        const size_t step = max_grid_blocks(TH);
+        for (size_t i = 0; i < nitems; i += step) {
+            const unsigned n = (unsigned)(nitems - i < step ? nitems - i : step);
+            if (flags & BDX_LAUNCH_STREAM)
+                hipLaunchKernelGGL(k_batch_tiny<true>, dim3(n), dim3(TH), 0, s, in, out, runs, items + i, tab);
+            else
+                hipLaunchKernelGGL(k_batch_tiny<false>, dim3(n), dim3(TH), 0, s, in, out, runs, items + i, tab);
+            if (hipGetLastError() != hipSuccess)
+                return -1;
+        }
+        return 0;
+    }
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(TH), 0, s, in, out, runs, items, units, nunits, qbase, ctr, tab,
+                       spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -14,6 +14,10 @@ constexpr int kThreads = 256;
 // XCD count of the current device, read once per device (the two-pass kernels order their
 // workgroups by XCD; each XCD has its own L2).
 unsigned xcd_count();
+// Most workgroups of `threads` lanes one launch may have: 2^32 - 1 work-items per grid
+// dimension, or the lower limit bdx_debug_grid_limit set.  Launchers whose grid grows with
+// the batch (whole-array kernels, the tiny batch kernel) split larger calls.
+size_t max_grid_blocks(int threads);
 // Floats of LDS for one workgroup of the whole-array kernel (144 KiB of the 160 KiB).
 constexpr int kSmallLds = BDX_SMALL_LDS_FLOATS;
 // Floats of LDS per workgroup of the row / column pass kernels (two workgroups per CU).

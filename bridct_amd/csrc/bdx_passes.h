@@ -111,6 +111,11 @@ __global__ __launch_bounds__(TH) void k_cols_narrow(const float *in, float *out,
     cols_body<H, MODE, G, false, TH>(in, out, w, offs, tab, xcds);
 }
 
+// Work-items per grid dimension of a launch.  The transform splits calls with more than
+// 512 MiB of intermediate into chunks of 256 MiB, far below this; single-pass calls
+// (bdx_transform_pass) are checked against it.
+constexpr size_t kMaxGrid = 0xffffffffu;
+
 // Workgroups below which a column pass takes narrower strips: two per CU.
 constexpr size_t kColsMinGrid = 512;
 
@@ -133,7 +138,7 @@ int rows_wg(const float *in, float *out, size_t nrows, int op, const float *tab,
             int h)
 {
     constexpr int R = row_unit_rows<W, kThreads>();
-    if (nrows % R || (G && (h <= 0 || h % R)))
+    if (nrows % R || (G && (h <= 0 || h % R)) || nrows / R > kMaxGrid / kThreads)
         return -1;
     const dim3 grid((unsigned)(nrows / R)), block(kThreads);
     switch (op) {
@@ -162,6 +167,8 @@ int cols_narrow(int w, const float *in, float *out, size_t batch, int mode, cons
                 const size_t *offs)
 {
     constexpr int L = col_unit_cols<H, TH>();
+    if (batch * (size_t)(w / L) > kMaxGrid / TH)
+        return -1;
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
     switch (mode) {
     case 0: hipLaunchKernelGGL((k_cols_narrow<H, 0, G, TH>), grid, block, 0, s, in, out, w, offs, tab, xcd_count()); break;
@@ -193,6 +200,8 @@ int cols_mode(int w, const float *in, float *out, size_t batch, const float *tab
             }
         }
     }
+    if (batch * (size_t)(w / L) > kMaxGrid / TH)
+        return -1;
     const dim3 grid((unsigned)(batch * (size_t)(w / L))), block(TH);
     hipLaunchKernelGGL((k_cols<H, MODE, G, NT>), grid, block, 0, s, in, out, w, offs, tab, xcd_count());
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -58,6 +58,8 @@ __global__ void k_zero(unsigned *p, unsigned n)
 // fault_after < 0: off; k >= 0: k more acquisitions succeed, the next one fails, then off.
 std::atomic<long> fault_after{-1};
 std::atomic<long> live_resources{0};
+// Workgroups per launch of the kernels that split large grids (0: the hardware limit).
+std::atomic<unsigned> grid_limit{0};
 
 // True when this acquisition must fail.
 bool injected_fault()
@@ -92,6 +94,13 @@ long bdx_debug_alloc_faults(long after)
 {
     fault_after.store(after < 0 ? -1 : after, std::memory_order_relaxed);
     return live_resources.load(std::memory_order_relaxed);
+}
+
+long bdx_debug_grid_limit(long blocks)
+{
+    const long prev = (long)grid_limit.load(std::memory_order_relaxed);
+    grid_limit.store(blocks > 0 && blocks <= 0x7fffffffL ? (unsigned)blocks : 0u, std::memory_order_relaxed);
+    return prev;
 }
 
 int bdx_device_ready(void)
@@ -340,3 +349,14 @@ int bdx_transform_gather(const float *d_in, float *d_out, int h, int w, size_t c
 }
 
 }  // extern "C"
+
+namespace bdx {
+
+size_t max_grid_blocks(int threads)
+{
+    // a launch covers at most 2^32 - 1 work-items per grid dimension
+    const size_t hw = 0xffffffffu / (unsigned)threads, cap = grid_limit.load(std::memory_order_relaxed);
+    return cap && cap < hw ? cap : hw;
+}
+
+}  // namespace bdx

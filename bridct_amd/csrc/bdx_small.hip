@@ -36,22 +36,31 @@ template <int H, int W, bool G>
 int launch_hwg(const float *in, float *out, size_t batch, size_t per, const size_t *offs, int mode,
                const float *tab, hipStream_t s)
 {
-    const size_t blocks = (batch + per - 1) / per;
     const size_t lds = per * H * (W + 1) * sizeof(float);
     const void *k = mode == 0 ? (const void *)k_small<H, W, 0, G>
                   : mode == 1 ? (const void *)k_small<H, W, 1, G> : (const void *)k_small<H, W, 2, G>;
     if (lds > 65536 && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_small<H, W, 0, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, in, out, (int)per,
-                           batch, offs, tab);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_small<H, W, 1, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, in, out, (int)per,
-                           batch, offs, tab);
-    else
-        hipLaunchKernelGGL((k_small<H, W, 2, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, in, out, (int)per,
-                           batch, offs, tab);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    // a batch whose grid would pass the launch limit runs in several launches
+    const size_t step = max_grid_blocks(kThreads) * per;
+    for (size_t a = 0; a < batch; a += step) {
+        const size_t n = batch - a < step ? batch - a : step, blocks = (n + per - 1) / per;
+        const float *i = G ? in : in + a * H * W;
+        float *o = G ? out : out + a * H * W;
+        const size_t *g = G ? offs + a : nullptr;
+        if (mode == 0)
+            hipLaunchKernelGGL((k_small<H, W, 0, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
+                               n, g, tab);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_small<H, W, 1, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
+                               n, g, tab);
+        else
+            hipLaunchKernelGGL((k_small<H, W, 2, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
+                               n, g, tab);
+        if (hipGetLastError() != hipSuccess)
+            return -1;
+    }
+    return 0;
 }
 
 template <int H, int W>

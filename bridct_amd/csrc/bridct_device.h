@@ -189,6 +189,10 @@ int bdx_current_device(int *device);
 int bdx_dense_static_work(int device, float **ptr, size_t *floats);
 /* Failure injection and resource count (bridct_debug_alloc_faults in bridct_gpu.h). */
 long bdx_debug_alloc_faults(long after);
+/* Test hook (bridct_debug_grid_limit in bridct_gpu.h): workgroups per launch of the
+ * launchers that split large grids; blocks <= 0 restores the hardware limit.  Returns the
+ * previous setting (0 for the hardware limit). */
+long bdx_debug_grid_limit(long blocks);
 int bdx_malloc(void **ptr, size_t bytes);
 int bdx_free(void *ptr);
 int bdx_host_malloc(void **ptr, size_t bytes);   /* pinned host memory */

@@ -653,6 +653,11 @@ long bridct_debug_alloc_faults(long after)
     return bdx_debug_alloc_faults(after);
 }
 
+long bridct_debug_grid_limit(long blocks)
+{
+    return bdx_debug_grid_limit(blocks);
+}
+
 void *bridct_gpu_malloc(size_t bytes)
 {
     void *p = NULL;

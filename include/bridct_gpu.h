@@ -193,6 +193,15 @@ BRIDCT_API int bridct_batch_get_strategy(const bridct_batch *batch);
  * Process-wide; not for concurrent use with other library calls. */
 BRIDCT_API long bridct_debug_alloc_faults(long after);
 
+/* Test hook for the split of large launches: the whole-array kernels and the tiny batch
+ * kernel launch at most 2^32 - 1 work-items per grid (the hardware limit; one 256-lane
+ * workgroup per 8 KiB or so of arrays, so calls from about 128 GiB on split).  blocks > 0
+ * lowers that to `blocks` workgroups per launch, so that a test can split small calls;
+ * blocks <= 0 restores the hardware limit.  Returns the previous setting (0: hardware
+ * limit).  Process-wide; not for concurrent use with other library calls.  No counterpart
+ * in the reference (its CPU loops have no grid). */
+BRIDCT_API long bridct_debug_grid_limit(long blocks);
+
 /* Device memory helpers (synchronous). */
 BRIDCT_API void *bridct_gpu_malloc(size_t bytes);
 BRIDCT_API int bridct_gpu_free(void *d_ptr);
