@@ -22,14 +22,15 @@
  * A second strategy, GROUPED, runs the arrays of each distinct shape with the plan path's
  * own per-shape kernels in gather form (the arrays of a shape are not contiguous in the
  * batch buffer, so the kernels read an offset table): one or two launches per shape,
- * largest shapes first, all on the batch stream and capturable as one graph.  Each launch
+ * largest shapes first, capturable as one graph.  Each launch
  * is specialized for its shape (registers, LDS, team size) and needs no cross-workgroup
  * dependency waits, which makes it faster when every shape has enough pixels to fill the
  * GPU; the automatic choice takes it when the average shape has at least
  * GROUPED_MIN_PIXELS pixels.  Batches of tiny arrays always use the tiny kernel.  The
- * whole-array shapes, whose launches are short and leave most of the GPU idle, go on a
- * second stream (forked from and joined back to the batch stream with events, so a
- * graph capture records both) and run beside the two-pass shapes.  GROUPED also follows
+ * large two-pass shapes run one after the other on the batch stream; the whole-array
+ * shapes and the small two-pass shapes, whose launches are short and leave most of the GPU
+ * idle, go on two auxiliary streams (forked from and joined back to the batch stream with
+ * events, so a graph capture records them all) and run beside them.  GROUPED also follows
  * the schedule table's route per shape: a shape the table sends to a split-MFMA route
  * (both axes >= 64) runs the dense GEMMs in gather form, so the automatic choice takes
  * GROUPED whenever such a shape is present (the persistent kernel runs butterflies only).
@@ -61,6 +62,16 @@
  * distinct shape (each launch then has several microseconds of work, so launch gaps stay
  * small against it). */
 #define GROUPED_MIN_PIXELS ((size_t)1 << 21)
+
+/* GROUPED: auxiliary streams beside the batch stream, and the pixel count below which a
+ * two-pass shape goes on one of them: its launches (a few hundred workgroups, 10-40 us)
+ * do not fill the GPU.  Measured on the 10,000-array mix (per direction): all two-pass
+ * shapes on the batch stream and one auxiliary stream for the whole-array shapes
+ * 2.16-2.18 ms; two auxiliary streams with two-pass shapes below 24 or 32 Mpx 1.93-1.96;
+ * below 16 Mpx 2.05; below 48 or 64 Mpx 1.98-2.01; three streams 1.94-2.01; one stream
+ * with the small two-pass shapes 2.18. */
+#define GROUPED_AUX_STREAMS 2
+#define GROUPED_SMALL_TWO_PASS ((size_t)32 << 20)
 
 /* One distinct shape of a batch: its arrays' offsets are d_goffs[first, first + count).
  * Dense shapes: Q_h', Q_h'^T, Q_w' at d_dense + qoff, and the exact scale 1 / (s_h s_w). */
@@ -99,8 +110,8 @@ struct bridct_batch {
     float *d_dense;          /* matrices of the dense shapes (GROUPED) */
     float *d_dwork;          /* their intermediate, one slice of the largest dense shape */
     int ndense;
-    void *aux_stream;        /* whole-array shapes of the grouped strategy */
-    void *ev_fork, *ev_join;
+    void *aux[GROUPED_AUX_STREAMS];       /* whole-array and small shapes of the grouped strategy */
+    void *ev_fork, *ev_join[GROUPED_AUX_STREAMS];
     host_stage *stage;       /* pinned staging of large host-pointer calls, on first use */
     void *stream;
     float *d_tables;
@@ -223,8 +234,9 @@ void bridct_batch_destroy(bridct_batch *b)
     /* both streams drained before any buffer they may still read is freed */
     if (b->stream)
         bdx_stream_sync(b->stream);
-    if (b->aux_stream)
-        bdx_stream_sync(b->aux_stream);
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
+        if (b->aux[k])
+            bdx_stream_sync(b->aux[k]);
     if (b->graph)
         bdx_graph_destroy(b->graph);
     stage_destroy(b->stage);
@@ -233,12 +245,14 @@ void bridct_batch_destroy(bridct_batch *b)
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             bdx_free(bufs[i]);
-    if (b->aux_stream)
-        bdx_stream_destroy(b->aux_stream);
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++) {
+        if (b->aux[k])
+            bdx_stream_destroy(b->aux[k]);
+        if (b->ev_join[k])
+            bdx_event_destroy(b->ev_join[k]);
+    }
     if (b->ev_fork)
         bdx_event_destroy(b->ev_fork);
-    if (b->ev_join)
-        bdx_event_destroy(b->ev_join);
     if (b->stream)
         bdx_stream_destroy(b->stream);
     free(b->offsets);
@@ -306,6 +320,17 @@ static int build_groups(bridct_batch *b, const int *heights, const int *widths, 
     }
     free(idx);
     qsort(b->groups, b->ngroups, sizeof(group), by_pixels_desc);
+    return 0;
+}
+
+/* The grouped strategy's auxiliary streams and their events. */
+static int aux_setup(bridct_batch *b)
+{
+    if (bdx_event_create(&b->ev_fork))
+        return -1;
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
+        if (bdx_stream_create(&b->aux[k]) || bdx_event_create(&b->ev_join[k]))
+            return -1;
     return 0;
 }
 
@@ -480,7 +505,7 @@ bridct_batch *bridct_batch_create(size_t count, const int *heights, const int *w
         upload((void **)&b->d_units, sc.units, sc.nunits * sizeof(bdx_unit), b->stream) ||
         upload((void **)&b->d_ctr, zeros, b->nctr * sizeof(unsigned), b->stream) ||
         upload((void **)&b->d_goffs, goffs, b->count * sizeof(size_t), b->stream) ||
-        bdx_stream_create(&b->aux_stream) || bdx_event_create(&b->ev_fork) || bdx_event_create(&b->ev_join) ||
+        aux_setup(b) ||
         (b->ndense && dense_setup(b)) || bdx_stream_sync(b->stream))
         goto fail;
     free(host_tables);
@@ -550,30 +575,55 @@ static int launch_dense(bridct_batch *b, const group *g, int mode, const float *
     return 0;
 }
 
-/* GROUPED: the plan path's kernels, per distinct shape, in gather form; two-pass shapes
- * on the batch stream, whole-array shapes on the auxiliary stream beside them. */
+/* GROUPED: the plan path's kernels, per distinct shape, in gather form.  Two-pass shapes of
+ * at least GROUPED_SMALL_TWO_PASS pixels go on the batch stream, one after the other;
+ * whole-array shapes and smaller two-pass shapes, whose launches leave most of the GPU
+ * idle, go on the auxiliary streams beside them (each to the one with the least work so
+ * far), forked from and joined back to the batch stream with events. */
+static int aux_group(const group *g)
+{
+    return g->route == BRIDCT_ROUTE_BUTTERFLY &&
+           (BDX_WHOLE_ARRAY(g->h, g->w, g->pack) || (size_t)g->h * g->w * g->count < GROUPED_SMALL_TWO_PASS);
+}
+
 static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
-    unsigned nsmall = 0;
+    size_t load[GROUPED_AUX_STREAMS] = {0};
+    int on_aux[GROUPED_AUX_STREAMS] = {0};
+    unsigned naux = 0;
     for (unsigned i = 0; i < b->ngroups; i++)
-        nsmall += BDX_WHOLE_ARRAY(b->groups[i].h, b->groups[i].w, b->groups[i].pack);
-    const int fork = nsmall > 0 && nsmall < b->ngroups;
-    if (fork && (bdx_event_record(b->ev_fork, b->stream) || bdx_stream_wait_event(b->aux_stream, b->ev_fork)))
-        return -1;
-    int rc = 0;
-    for (unsigned i = 0; i < b->ngroups && !rc; i++) {
+        naux += aux_group(&b->groups[i]);
+    /* a batch with nothing for the batch stream keeps everything on it */
+    const int fork = naux > 0 && naux < b->ngroups;
+    int rc = fork ? bdx_event_record(b->ev_fork, b->stream) : 0;
+    for (unsigned i = 0; i < b->ngroups && !rc; i++) {   /* largest first */
         const group *g = &b->groups[i];
         if (g->route != BRIDCT_ROUTE_BUTTERFLY) {
             rc = launch_dense(b, g, mode, d_in, d_out);
             continue;
         }
-        void *s = fork && BDX_WHOLE_ARRAY(g->h, g->w, g->pack) ? b->aux_stream : b->stream;
-        rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
-                                  b->d_tables, s);
+        const int whole = BDX_WHOLE_ARRAY(g->h, g->w, g->pack);
+        const size_t px = (size_t)g->h * g->w * g->count;
+        void *s = b->stream;
+        if (fork && aux_group(g)) {
+            int k = 0;
+            for (int j = 1; j < GROUPED_AUX_STREAMS; j++)
+                if (load[j] < load[k])
+                    k = j;
+            if (!on_aux[k] && !rc)
+                rc = bdx_stream_wait_event(b->aux[k], b->ev_fork);
+            on_aux[k] = 1;
+            load[k] += (whole ? 1 : 2) * (px + ((size_t)1 << 21));   /* a launch costs about 2 Mpx */
+            s = b->aux[k];
+        }
+        if (!rc)
+            rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
+                                      b->d_tables, s);
     }
     /* joined even after a failed launch, so that the batch stream covers everything issued */
-    if (fork && (bdx_event_record(b->ev_join, b->aux_stream) || bdx_stream_wait_event(b->stream, b->ev_join)))
-        rc = -1;
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
+        if (on_aux[k] && (bdx_event_record(b->ev_join[k], b->aux[k]) || bdx_stream_wait_event(b->stream, b->ev_join[k])))
+            rc = -1;
     return rc;
 }
 
@@ -646,7 +696,10 @@ int bridct_batch_synchronize(bridct_batch *b)
     if (!b)
         return BRIDCT_INVALID_ARGUMENT;
     unsigned err = 0;
-    if (bdx_stream_sync(b->stream) || bdx_stream_sync(b->aux_stream) ||
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
+        if (bdx_stream_sync(b->aux[k]))
+            return BRIDCT_DEVICE_ERROR;
+    if (bdx_stream_sync(b->stream) ||
         bdx_copy_d2h(&err, b->d_ctr + BDX_CTR_ERROR, sizeof(err), b->stream) || bdx_stream_sync(b->stream))
         return BRIDCT_DEVICE_ERROR;
     if (err) {   /* reported once: grouped calls do not reset the flag themselves */
