@@ -579,7 +579,8 @@ static int launch_dense(bridct_batch *b, const group *g, int mode, const float *
  * at least GROUPED_SMALL_TWO_PASS pixels go on the batch stream, one after the other;
  * whole-array shapes and smaller two-pass shapes, whose launches leave most of the GPU
  * idle, go on the auxiliary streams beside them (each to the one with the least work so
- * far), forked from and joined back to the batch stream with events. */
+ * far; the batch stream takes its share too when no shape is that large), forked from and
+ * joined back to the batch stream with events. */
 static int aux_group(const group *g)
 {
     return g->route == BRIDCT_ROUTE_BUTTERFLY &&
@@ -588,13 +589,21 @@ static int aux_group(const group *g)
 
 static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d_out)
 {
-    size_t load[GROUPED_AUX_STREAMS] = {0};
-    int on_aux[GROUPED_AUX_STREAMS] = {0};
     unsigned naux = 0;
     for (unsigned i = 0; i < b->ngroups; i++)
         naux += aux_group(&b->groups[i]);
-    /* a batch with nothing for the batch stream keeps everything on it */
-    const int fork = naux > 0 && naux < b->ngroups;
+    /* Streams the small groups are dealt to: the auxiliary ones, and the batch stream too
+     * when no group needs it for itself. */
+    void *cand[GROUPED_AUX_STREAMS + 1];
+    const int first_aux = naux == b->ngroups;   /* index of aux[0] in cand */
+    int nc = 0;
+    if (first_aux)
+        cand[nc++] = b->stream;
+    for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
+        cand[nc++] = b->aux[k];
+    size_t load[GROUPED_AUX_STREAMS + 1] = {0};
+    int used[GROUPED_AUX_STREAMS + 1] = {0};
+    const int fork = naux > 0;
     int rc = fork ? bdx_event_record(b->ev_fork, b->stream) : 0;
     for (unsigned i = 0; i < b->ngroups && !rc; i++) {   /* largest first */
         const group *g = &b->groups[i];
@@ -602,27 +611,29 @@ static int launch_grouped(bridct_batch *b, int mode, const float *d_in, float *d
             rc = launch_dense(b, g, mode, d_in, d_out);
             continue;
         }
-        const int whole = BDX_WHOLE_ARRAY(g->h, g->w, g->pack);
-        const size_t px = (size_t)g->h * g->w * g->count;
         void *s = b->stream;
-        if (fork && aux_group(g)) {
+        if (aux_group(g)) {
+            const int whole = BDX_WHOLE_ARRAY(g->h, g->w, g->pack);
+            const size_t px = (size_t)g->h * g->w * g->count;
             int k = 0;
-            for (int j = 1; j < GROUPED_AUX_STREAMS; j++)
+            for (int j = 1; j < nc; j++)
                 if (load[j] < load[k])
                     k = j;
-            if (!on_aux[k] && !rc)
-                rc = bdx_stream_wait_event(b->aux[k], b->ev_fork);
-            on_aux[k] = 1;
+            s = cand[k];
+            if (!used[k] && s != b->stream)
+                rc = bdx_stream_wait_event(s, b->ev_fork);
+            used[k] = 1;
             load[k] += (whole ? 1 : 2) * (px + ((size_t)1 << 21));   /* a launch costs about 2 Mpx */
-            s = b->aux[k];
         }
         if (!rc)
             rc = bdx_transform_gather(d_in, d_out, g->h, g->w, g->count, b->d_goffs + g->first, g->pack, mode,
                                       b->d_tables, s);
     }
-    /* joined even after a failed launch, so that the batch stream covers everything issued */
+    /* the auxiliary streams this call used are joined, even after a failed launch, so that
+     * the batch stream covers everything issued */
     for (int k = 0; k < GROUPED_AUX_STREAMS; k++)
-        if (on_aux[k] && (bdx_event_record(b->ev_join[k], b->aux[k]) || bdx_stream_wait_event(b->stream, b->ev_join[k])))
+        if (used[first_aux + k] &&
+            (bdx_event_record(b->ev_join[k], b->aux[k]) || bdx_stream_wait_event(b->stream, b->ev_join[k])))
             rc = -1;
     return rc;
 }
