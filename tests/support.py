@@ -9,6 +9,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# Where GPU tests write their result tables (error per shape, corpus gate summary).
+OUT_DIR = os.environ.get('BRIDCT_TEST_OUT', os.path.join(ROOT, 'test_out'))
 ORACLE_LIB = os.path.join(ROOT, 'oracle', 'liboracle.so')
 REF_LIB = os.path.join(ROOT, 'oracle', '_ref', 'libbridct.so')
 LENGTHS = (8, 16, 32, 64, 128, 256, 512, 1024)

@@ -60,8 +60,8 @@ def test_corpus_gate():
         boundary_failures_cpu=sum(not r['cpu_pass'] for r in boundary),
         cpu_pass_gpu_fail=[(r['id'], r['mode']) for r in rows if r['cpu_pass'] and not r['gpu_pass']],
         gpu_pass_cpu_fail=sum(1 for r in rows if r['gpu_pass'] and not r['cpu_pass']))
-    os.makedirs(os.path.join(S.ROOT, 'gpurun_out'), exist_ok=True)
-    with open(os.path.join(S.ROOT, 'gpurun_out', 'corpus_gpu.json'), 'w') as f:
+    os.makedirs(S.OUT_DIR, exist_ok=True)
+    with open(os.path.join(S.OUT_DIR, 'corpus_gpu.json'), 'w') as f:
         json.dump(dict(summary=summary, rows=rows), f, indent=0)
     print(json.dumps(summary), flush=True)
     assert len(rows) == 1422 and summary['core_directions'] == 1260

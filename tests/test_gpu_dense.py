@@ -148,7 +148,7 @@ def test_dense_route_rules():
 
 
 def test_write_dense_error_table():
-    out = os.path.join(S.ROOT, 'gpurun_out')
+    out = S.OUT_DIR
     os.makedirs(out, exist_ok=True)
     with open(os.path.join(out, 'dense_errors.json'), 'w') as f:
         json.dump(ERRORS, f, indent=1, sort_keys=True)

@@ -18,7 +18,7 @@ from bridct_amd import capi
 
 pytestmark = pytest.mark.gpu
 
-OUT = os.path.join(S.ROOT, 'gpurun_out')
+OUT = S.OUT_DIR
 
 
 def gpu_apply(plan, x, mode):
