@@ -45,18 +45,18 @@ int launch_hwg(const float *in, float *out, size_t batch, size_t per, const size
     const size_t step = max_grid_blocks(kThreads) * per;
     for (size_t a = 0; a < batch; a += step) {
         const size_t n = batch - a < step ? batch - a : step, blocks = (n + per - 1) / per;
-        const float *i = G ? in : in + a * H * W;
-        float *o = G ? out : out + a * H * W;
-        const size_t *g = G ? offs + a : nullptr;
+        const float *src = G ? in : in + a * H * W;
+        float *dst = G ? out : out + a * H * W;
+        const size_t *og = G ? offs + a : nullptr;
         if (mode == 0)
-            hipLaunchKernelGGL((k_small<H, W, 0, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
-                               n, g, tab);
+            hipLaunchKernelGGL((k_small<H, W, 0, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, src, dst,
+                               (int)per, n, og, tab);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_small<H, W, 1, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
-                               n, g, tab);
+            hipLaunchKernelGGL((k_small<H, W, 1, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, src, dst,
+                               (int)per, n, og, tab);
         else
-            hipLaunchKernelGGL((k_small<H, W, 2, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, i, o, (int)per,
-                               n, g, tab);
+            hipLaunchKernelGGL((k_small<H, W, 2, G>), dim3((unsigned)blocks), dim3(kThreads), lds, s, src, dst,
+                               (int)per, n, og, tab);
         if (hipGetLastError() != hipSuccess)
             return -1;
     }
