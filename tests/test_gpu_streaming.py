@@ -109,7 +109,8 @@ def test_1024_chunked_batch_bitwise(in_place):
 def test_grouped_chunked_gather_bitwise():
     """A ragged batch with 140 arrays of 1024^2 (560 MiB of intermediate in that group: gather
     calls in chunks of 64, 64 and 12) and a few 256x512 and 64x64 fillers, grouped strategy:
-    every array bitwise equal to a plan call on 16 arrays of its shape, both directions."""
+    every array bitwise equal to a plan call on 16 arrays of its shape, forward, inverse and
+    fused round trip."""
     shapes = [(1024, 1024)] * 70 + [(256, 512)] * 5 + [(1024, 1024)] * 70 + [(64, 64)] * 7
     b = capi.Batch(shapes)
     try:
@@ -117,7 +118,7 @@ def test_grouped_chunked_gather_bitwise():
         x = np.random.default_rng(140).uniform(-1, 1, b.elements).astype(np.float32)
         d_x, d_y = capi.DeviceBuffer(x.nbytes), capi.DeviceBuffer(x.nbytes)
         d_x.upload(x)
-        for mode in (capi.FORWARD, capi.INVERSE):
+        for mode in (capi.FORWARD, capi.INVERSE, capi.ROUNDTRIP):
             print('grouped 140 x 1024^2 chunked, mode %d' % mode, flush=True)
             assert b.apply_device(mode, d_x.ptr, d_y.ptr) == 0 and b.synchronize() == 0
             y = np.empty_like(x)
